@@ -1,0 +1,96 @@
+// Internal declarations shared by the host driver and the HIP kernels of
+// libdpca.  Nothing here is part of the C ABI (include/dpca.h is).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#define DPCA_CHECK(expr)                                                          \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            dpca::set_error("%s:%d: %s failed: %s", __FILE__, __LINE__, #expr,    \
+                            hipGetErrorString(e_));                               \
+            return -2;                                                            \
+        }                                                                         \
+    } while (0)
+
+namespace dpca {
+
+void set_error(const char *fmt, ...);
+
+// ---------------------------------------------------------------- layout --
+// Packed genotypes live in HBM SNP-major like the .bed file, but each row is
+// padded to `ldp` bytes (a multiple of 64, i.e. 256 samples) and the row count
+// to a multiple of 256.  Padding codes are 01 (missing), so they decode to 0
+// in both numerical forms and no kernel needs a bounds check.
+constexpr int kRowAlign = 256;    // SNP rows padded to this multiple
+constexpr int kLdpAlign = 64;     // bytes per padded row: multiple of this
+constexpr uint8_t kPadByte = 0x55;
+
+inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+
+// -------------------------------------------------------------- kernels --
+// Every launcher enqueues on `st` and returns hipGetLastError().
+
+// raw .bed rows (np bytes each) -> padded rows (ldp bytes), tail codes of
+// the last real byte set to 01.
+hipError_t launch_repack(hipStream_t st, const uint8_t *raw, int64_t rows, int64_t np,
+                         uint8_t *dst, int64_t ldp, int M);
+hipError_t launch_fill_u8(hipStream_t st, uint8_t *p, int64_t n, uint8_t v);
+hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v);
+
+// Per-SNP integer statistics and the FP64 value table zval[4*s + code].
+// Also the mixed-form per-SNP mean mu[s] and scale dsc[s] = 1/(sd*sqrt(N))
+// (0 where the reference's sd > 1e-9 guard fails).
+hipError_t launch_snp_stats(hipStream_t st, const uint8_t *packed, int64_t rows,
+                            int64_t ldp, double inv_sqrtN, int64_t *sum, int64_t *ctr,
+                            double *zval, double *mu, double *dsc);
+
+// Decode rows to reference codes {0,1,2,3=missing} (first-touch table).
+hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t rows,
+                               int64_t ldp, int M, uint8_t *codes);
+
+// FP64 form, T = Z Q over all padded rows.  T: rows x kp.
+hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                         const double *zval, const double *Q, int kp, double *T);
+// FP64 form, partial[sp] = Z^T T over split sp of the rows.
+hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                         const double *zval, const double *T, int kp, int splits,
+                         double *part);
+// out (+)= sum_sp part[sp]; n = Mpad*kp
+hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
+                               double *out, int accumulate);
+
+// Mixed form (bf16 hi+lo operands, FP32 accumulate).
+hipError_t launch_split_bf16(hipStream_t st, const double *Q, int64_t rows, int kp,
+                             const double *rowscale, uint32_t *hl);
+hipError_t launch_za_bf16(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                          const uint32_t *Qhl, int kp, float *HQ);
+hipError_t launch_zt_bf16(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                          const uint32_t *Uhl, int kp, int splits, float *part);
+hipError_t launch_mask_fix_a(hipStream_t st, const float *HQ, const double *mu,
+                             const double *dsc, const double *colsumQ,
+                             const float *missQ, int64_t rows, int kp, double *T);
+
+// Tall-skinny helpers, FP64.  X, Y: m x kp row-major.
+hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
+                       double *part, int nparts, double *G);
+hipError_t launch_chol_inv(hipStream_t st, const double *G, int k, int kp, double shift_mult,
+                           int64_t m, double *Rinv, int *status);
+hipError_t launch_apply(hipStream_t st, const double *X, const double *W, int64_t m, int kp,
+                        double *Y);
+hipError_t launch_syev(hipStream_t st, const double *B, int k, int kp, double *w, double *V,
+                       int *status);
+hipError_t launch_col_dots(hipStream_t st, const double *X, const double *Y, int64_t m,
+                           int kp, int ncols, double *part, int nparts, double *dots);
+hipError_t launch_colsum(hipStream_t st, const double *X, int64_t m, int kp, double *part,
+                         int nparts, double *out);
+
+// Synthetic generator (device side).  See dpca_gen.h.
+struct GenSpec;
+hipError_t launch_generate(hipStream_t st, const GenSpec &g, const uint32_t *thr,
+                           const uint16_t *wts, int64_t snp0, int64_t rows, uint8_t *dst,
+                           int64_t ldp);
+
+}  // namespace dpca

@@ -1,0 +1,42 @@
+// Device-side helpers shared by the HIP kernels of libdpca (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dpca {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef short s8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+
+// 64-wide wavefront sum.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ long long wave_sum_ll(long long v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// 2-bit code of sample t (0..63) within 16 packed bytes held as 4 dwords.
+__device__ __forceinline__ uint32_t code_at(const uint32_t (&w)[4], int t)
+{
+    return (w[t >> 4] >> (2 * (t & 15))) & 3u;
+}
+
+// Select one of four doubles by a 2-bit code without indexing registers.
+__device__ __forceinline__ double sel4(uint32_t c, double v0, double v1, double v2,
+                                       double v3)
+{
+    double lo = (c & 1u) ? v1 : v0;
+    double hi = (c & 1u) ? v3 : v2;
+    return (c & 2u) ? hi : lo;
+}
+
+}  // namespace dpca

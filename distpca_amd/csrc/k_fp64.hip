@@ -1,0 +1,346 @@
+// Packed-genotype kernels of libdpca: repack into the padded HBM layout,
+// per-SNP statistics with the reference's value table, decode, and the FP64
+// fused decode + MFMA products of one pass  A A^T Q = Z^T (Z Q).
+//
+// Z is never materialised: each kernel reads the 2-bit codes and turns them
+// into Z values through the per-SNP 4-entry table zval (exactly the values the
+// reference's decode + standardise produce, src/methods.cpp:814-845).
+#include "dpca_internal.h"
+#include "k_common.h"
+
+namespace dpca {
+
+// ------------------------------------------------------------------ repack --
+// raw rows of np bytes (the .bed layout after the 3-byte magic) -> rows of ldp
+// bytes.  Bytes past np are 0x55 (four missing codes); within the last real
+// byte, the codes of samples >= M are forced to 01 (missing) so that padding
+// never enters any sum.  One thread per destination byte.
+__global__ void k_repack(const uint8_t *__restrict__ raw, int64_t rows, int64_t np,
+                         uint8_t *__restrict__ dst, int64_t ldp, int M)
+{
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t total = rows * ldp;
+    if (idx >= total) return;
+    int64_t r = idx / ldp, b = idx - r * ldp;
+    uint8_t v = kPadByte;
+    if (b < np) {
+        v = raw[r * np + b];
+        if (b == np - 1) {
+            int valid = M - 4 * (int)(np - 1);  // 1..4 real samples in this byte
+            if (valid < 4) {
+                uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
+                v = (uint8_t)((v & keep) | (kPadByte & ~keep));
+            }
+        }
+    }
+    dst[idx] = v;
+}
+
+__global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f64(double *p, int64_t n, double v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+hipError_t launch_repack(hipStream_t st, const uint8_t *raw, int64_t rows, int64_t np,
+                         uint8_t *dst, int64_t ldp, int M)
+{
+    int64_t total = rows * ldp;
+    if (total == 0) return hipSuccess;
+    k_repack<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(raw, rows, np, dst, ldp, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u8(hipStream_t st, uint8_t *p, int64_t n, uint8_t v)
+{
+    if (n == 0) return hipSuccess;
+    k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v)
+{
+    if (n == 0) return hipSuccess;
+    k_fill_f64<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, v);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------- SNP stats --
+// One wavefront per SNP row.  Codes are counted 16 at a time with popcounts:
+// with lo = bit0 and hi = bit1 of every 2-bit field, missing (01) = lo & ~hi,
+// and sum g = popc(hi) + popc(hi & lo) since 10 -> 1 and 11 -> 2.  Integer
+// counts are exact; the table is then formed in the reference's own IEEE
+// operation order (src/utilities.cpp:288-301, src/methods.cpp:824-830, with
+// the * inv_sqrtN of src/methods.cpp:842 folded in).
+__global__ __launch_bounds__(256) void k_snp_stats(const uint8_t *__restrict__ packed,
+                                                   int64_t rows, int64_t ldp,
+                                                   double inv_sqrtN, int64_t *sum,
+                                                   int64_t *ctr, double *zval, double *mu,
+                                                   double *dsc)
+{
+#pragma clang fp contract(off)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t s = (int64_t)blockIdx.x * 4 + wave;
+    if (s >= rows) return;
+    const uint4 *row = reinterpret_cast<const uint4 *>(packed + s * ldp);
+    const int nq = (int)(ldp >> 4);
+    long long sm = 0, c = 0;
+    for (int q = lane; q < nq; q += 64) {
+        uint4 v = row[q];
+        uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            uint32_t lo = w[e] & 0x55555555u, hi = (w[e] >> 1) & 0x55555555u;
+            uint32_t miss = lo & ~hi;
+            c += __popc(0x55555555u & ~miss);
+            sm += __popc(hi) + __popc(hi & lo);
+        }
+    }
+    sm = wave_sum_ll(sm);
+    c = wave_sum_ll(c);
+    if (lane == 0) {
+        double a = (double)sm / (double)(unsigned int)c;
+        double P = a / 2.0;
+        double d = sqrt(2.0 * P * (1 - P));
+        sum[s] = sm;
+        ctr[s] = c;
+        double z0 = 0, z2 = 0, z3 = 0, m = 0, sc = 0;
+        if (d > 1e-9) {
+            z0 = (0 - a) / d * inv_sqrtN;
+            z2 = (1 - a) / d * inv_sqrtN;
+            z3 = (2 - a) / d * inv_sqrtN;
+            m = a;
+            sc = inv_sqrtN / d;
+        }
+        zval[4 * s + 0] = z0;
+        zval[4 * s + 1] = 0;
+        zval[4 * s + 2] = z2;
+        zval[4 * s + 3] = z3;
+        mu[s] = m;
+        dsc[s] = sc;
+    }
+}
+
+hipError_t launch_snp_stats(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                            double inv_sqrtN, int64_t *sum, int64_t *ctr, double *zval,
+                            double *mu, double *dsc)
+{
+    if (rows == 0) return hipSuccess;
+    k_snp_stats<<<(unsigned)((rows + 3) / 4), 256, 0, st>>>(packed, rows, ldp, inv_sqrtN, sum,
+                                                            ctr, zval, mu, dsc);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ decode --
+// codes[s*M + i] = {0,3,1,2}[2-bit code]  (src/utilities.cpp:353).
+__global__ void k_decode_codes(const uint8_t *__restrict__ packed, int64_t rows, int64_t ldp,
+                               int M, uint8_t *__restrict__ codes)
+{
+    int64_t nb = (M + 3) / 4;
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows * nb) return;
+    int64_t s = idx / nb, b = idx - s * nb;
+    uint8_t v = packed[s * ldp + b];
+    const uint8_t map[4] = {0, 3, 1, 2};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int64_t i = 4 * b + j;
+        if (i < M) codes[s * M + i] = map[(v >> (2 * j)) & 3];
+    }
+}
+
+hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t rows,
+                               int64_t ldp, int M, uint8_t *codes)
+{
+    int64_t n = rows * ((M + 3) / 4);
+    if (n == 0) return hipSuccess;
+    k_decode_codes<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(packed, rows, ldp, M, codes);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- FP64  T = Z Q --
+// v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16).  Lane l = 16*g + r
+// supplies A[r][g] and B[g][r] and holds D[4g + q][r], q = 0..3.
+// Rows of the tile are SNPs, columns are right-hand sides, and the 4-deep K
+// slot g is a sample.  Which sample sits in which K slot is free as long as
+// A and B agree: lane group g walks the 64 samples of its own 16 packed
+// bytes, so every lane decodes from one 16-byte load per 256-sample chunk.
+// A workgroup owns 128 SNPs (4 waves x 2 tiles of 16) and the whole sample
+// range; Q (Mpad x kp, zero rows past M) is read through L1/L2.
+template <int CT>
+__global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ packed,
+                                                int64_t ldp, const double *__restrict__ zval,
+                                                const double *__restrict__ Q,
+                                                double *__restrict__ T)
+{
+    constexpr int KP = CT * 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int64_t base = (int64_t)blockIdx.x * 128 + wave * 32;
+    const int64_t s0 = base + r, s1 = base + 16 + r;
+    const double a00 = zval[4 * s0 + 0], a01 = zval[4 * s0 + 1], a02 = zval[4 * s0 + 2],
+                 a03 = zval[4 * s0 + 3];
+    const double a10 = zval[4 * s1 + 0], a11 = zval[4 * s1 + 1], a12 = zval[4 * s1 + 2],
+                 a13 = zval[4 * s1 + 3];
+    d4 acc0[CT], acc1[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        acc0[c] = d4{0, 0, 0, 0};
+        acc1[c] = d4{0, 0, 0, 0};
+    }
+    const uint8_t *p0 = packed + s0 * ldp + 16 * g;
+    const uint8_t *p1 = packed + s1 * ldp + 16 * g;
+    for (int64_t cb = 0; cb < ldp; cb += 64) {
+        const uint4 v0 = *reinterpret_cast<const uint4 *>(p0 + cb);
+        const uint4 v1 = *reinterpret_cast<const uint4 *>(p1 + cb);
+        const uint32_t w0[4] = {v0.x, v0.y, v0.z, v0.w};
+        const uint32_t w1[4] = {v1.x, v1.y, v1.z, v1.w};
+        const double *qrow = Q + (4 * cb + 64 * g) * KP + r;
+#pragma unroll 8
+        for (int t = 0; t < 64; t++) {
+            double b[CT];
+#pragma unroll
+            for (int c = 0; c < CT; c++) b[c] = qrow[t * KP + c * 16];
+            const uint32_t c0 = code_at(w0, t), c1 = code_at(w1, t);
+            const double x0 = sel4(c0, a00, a01, a02, a03);
+            const double x1 = sel4(c1, a10, a11, a12, a13);
+#pragma unroll
+            for (int c = 0; c < CT; c++) {
+                acc0[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, b[c], acc0[c], 0, 0, 0);
+                acc1[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, b[c], acc1[c], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CT; c++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            T[(base + 4 * g + q) * KP + c * 16 + r] = acc0[c][q];
+            T[(base + 16 + 4 * g + q) * KP + c * 16 + r] = acc1[c][q];
+        }
+}
+
+hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                         const double *zval, const double *Q, int kp, double *T)
+{
+    if (rows % 128 != 0) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(rows / 128));
+    switch (kp) {
+    case 16: k_za_f64<1><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
+    case 32: k_za_f64<2><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
+    case 48: k_za_f64<3><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
+    case 64: k_za_f64<4><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- FP64  Z^T T part --
+// D(16 samples x 16 rhs) += A(16 samples x 4 SNPs) B(4 SNPs x 16 rhs).
+// A workgroup owns 256 samples (64 bytes of every row) and a contiguous SNP
+// range (split sp); each wave owns 64 samples = 4 tiles.  64-SNP chunks of
+// packed bytes, value tables and T rows are staged in LDS.
+template <int CT>
+__global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ packed,
+                                                int64_t rows, int64_t rows_per_split,
+                                                int64_t ldp, const double *__restrict__ zval,
+                                                const double *__restrict__ T,
+                                                double *__restrict__ part, int64_t Mpad)
+{
+    constexpr int KP = CT * 16;
+    __shared__ uint8_t pk[64][64];
+    __shared__ double zl[64][4];
+    __shared__ double tl[64][KP];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int64_t mb = (int64_t)blockIdx.x * 64;  // byte offset of this sample tile
+    const int sp = blockIdx.y;
+    const int64_t sbeg = sp * rows_per_split;
+    int64_t send = sbeg + rows_per_split;
+    if (send > rows) send = rows;
+    d4 acc[4][CT];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < CT; c++) acc[a][c] = d4{0, 0, 0, 0};
+
+    for (int64_t s0 = sbeg; s0 < send; s0 += 64) {
+        __syncthreads();
+        {
+            const int rr = tid >> 2, pp = tid & 3;
+            *reinterpret_cast<uint4 *>(&pk[rr][pp * 16]) =
+                *reinterpret_cast<const uint4 *>(packed + (s0 + rr) * ldp + mb + pp * 16);
+            zl[rr][pp] = zval[4 * (s0 + rr) + pp];
+        }
+        for (int e = tid; e < 64 * KP; e += 256) tl[e / KP][e % KP] = T[s0 * KP + e];
+        __syncthreads();
+#pragma unroll 4
+        for (int step = 0; step < 16; step++) {
+            const int sl = 4 * step + g;
+            double b[CT];
+#pragma unroll
+            for (int c = 0; c < CT; c++) b[c] = tl[sl][c * 16 + r];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int i = wave * 64 + a * 16 + r;
+                const uint32_t code = (pk[sl][i >> 2] >> (2 * (i & 3))) & 3u;
+                const double x = zl[sl][code];
+#pragma unroll
+                for (int c = 0; c < CT; c++)
+                    acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, b[c], acc[a][c], 0, 0, 0);
+            }
+        }
+    }
+    double *out = part + ((int64_t)sp * Mpad + 4 * mb + wave * 64) * KP;
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int c = 0; c < CT; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) out[(a * 16 + 4 * g + q) * KP + c * 16 + r] = acc[a][c][q];
+}
+
+hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                         const double *zval, const double *T, int kp, int splits, double *part)
+{
+    if (rows % 64 != 0 || ldp % 64 != 0 || splits < 1) return hipErrorInvalidValue;
+    int64_t rps = round_up((rows + splits - 1) / splits, 64);
+    dim3 grid((unsigned)(ldp / 64), (unsigned)splits);
+    int64_t Mpad = 4 * ldp;
+    switch (kp) {
+    case 16: k_zt_f64<1><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
+    case 32: k_zt_f64<2><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
+    case 48: k_zt_f64<3><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
+    case 64: k_zt_f64<4><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// out[e] (+)= sum over splits, in split order (deterministic).
+__global__ void k_reduce_parts(const double *__restrict__ part, int splits, int64_t n,
+                               double *__restrict__ out, int accumulate)
+{
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = accumulate ? out[e] : 0.0;
+    for (int p = 0; p < splits; p++) s += part[(int64_t)p * n + e];
+    out[e] = s;
+}
+
+hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
+                               double *out, int accumulate)
+{
+    if (n == 0) return hipSuccess;
+    k_reduce_parts<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, n, out,
+                                                                accumulate);
+    return hipGetLastError();
+}
+
+}  // namespace dpca

@@ -1,0 +1,374 @@
+// Small dense FP64 kernels of the subspace iteration: tall-skinny Gram
+// products, Cholesky + triangular inverse (CholQR2), the M x k by k x k
+// product, the k x k symmetric eigensolver (parallel cyclic Jacobi) and the
+// column dot products of the MEV criterion.  All reductions run in a fixed
+// order, so results do not depend on scheduling.  k <= 64 throughout.
+#include "dpca_internal.h"
+#include "k_common.h"
+
+namespace dpca {
+
+constexpr int kMaxKp = 64;
+
+// ------------------------------------------------------------------- Gram --
+// part[p][a*kp + b] = sum_{i in rows of p} X[i][a] * Y[i][b]; 32-row tiles of
+// X and Y are staged in LDS; thread t owns entries t, t+256, ...
+template <int KP>
+__global__ __launch_bounds__(256) void k_gram_part(const double *__restrict__ X,
+                                                   const double *__restrict__ Y, int64_t m,
+                                                   int64_t rpp, double *__restrict__ part)
+{
+    constexpr int NE = (KP * KP + 255) / 256;
+    __shared__ double xs[32][KP + 1];
+    __shared__ double ys[32][KP + 1];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * rpp;
+    int64_t r1 = r0 + rpp;
+    if (r1 > m) r1 = m;
+    double acc[NE];
+#pragma unroll
+    for (int u = 0; u < NE; u++) acc[u] = 0;
+    for (int64_t rb = r0; rb < r1; rb += 32) {
+        __syncthreads();
+        for (int e = tid; e < 32 * KP; e += 256) {
+            const int rr = e / KP, cc = e % KP;
+            const int64_t i = rb + rr;
+            xs[rr][cc] = (i < r1) ? X[i * KP + cc] : 0.0;
+            ys[rr][cc] = (i < r1) ? Y[i * KP + cc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NE; u++) {
+            const int e = tid + 256 * u;
+            if (e < KP * KP) {
+                const int a = e / KP, b = e % KP;
+                double s = acc[u];
+#pragma unroll 8
+                for (int rr = 0; rr < 32; rr++) s += xs[rr][a] * ys[rr][b];
+                acc[u] = s;
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NE; u++) {
+        const int e = tid + 256 * u;
+        if (e < KP * KP) part[(int64_t)blockIdx.x * KP * KP + e] = acc[u];
+    }
+}
+
+__global__ void k_sum_parts(const double *__restrict__ part, int nparts, int n,
+                            double *__restrict__ out)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = 0;
+    for (int p = 0; p < nparts; p++) s += part[(int64_t)p * n + e];
+    out[e] = s;
+}
+
+hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
+                       double *part, int nparts, double *G)
+{
+    int64_t rpp = (m + nparts - 1) / nparts;
+    switch (kp) {
+    case 16: k_gram_part<16><<<nparts, 256, 0, st>>>(X, Y, m, rpp, part); break;
+    case 32: k_gram_part<32><<<nparts, 256, 0, st>>>(X, Y, m, rpp, part); break;
+    case 48: k_gram_part<48><<<nparts, 256, 0, st>>>(X, Y, m, rpp, part); break;
+    case 64: k_gram_part<64><<<nparts, 256, 0, st>>>(X, Y, m, rpp, part); break;
+    default: return hipErrorInvalidValue;
+    }
+    int n = kp * kp;
+    k_sum_parts<<<(n + 255) / 256, 256, 0, st>>>(part, nparts, n, G);
+    return hipGetLastError();
+}
+
+// -------------------------------------------------- Cholesky + R^{-1} ------
+// G (kp x kp, leading k x k used) = R^T R with R upper triangular; writes
+// R^{-1} (kp x kp, zero outside the leading block).  With shift_mult > 0 the
+// diagonal is first raised by shift_mult * trace(G) (shifted CholeskyQR,
+// Fukaya et al., SIAM J. Sci. Comput. 42 (2020)).  status = 1 when a pivot
+// is not positive.
+__global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, int k,
+                                                  int kp, double shift_mult,
+                                                  double *__restrict__ Rinv, int *status)
+{
+    __shared__ double A[kMaxKp][kMaxKp + 1];
+    __shared__ double col[kMaxKp];
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < kp * kp; e += 256) {
+        const int i = e / kp, j = e % kp;
+        // the upper triangle defines the symmetric matrix
+        A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
+    }
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    if (shift_mult > 0) {
+        if (tid == 0) {
+            double tr = 0;
+            for (int i = 0; i < k; i++) tr += A[i][i];
+            const double sh = shift_mult * tr;
+            for (int i = 0; i < k; i++) A[i][i] += sh;
+        }
+        __syncthreads();
+    }
+    for (int j = 0; j < k; j++) {
+        if (tid == 0) {
+            const double d = A[j][j];
+            if (!(d > 0.0)) bad = 1;
+            A[j][j] = sqrt(d > 0.0 ? d : 1.0);
+        }
+        __syncthreads();
+        const double piv = A[j][j];
+        for (int c = j + 1 + tid; c < k; c += 256) A[j][c] /= piv;
+        __syncthreads();
+        const int n = k - j - 1;
+        for (int e = tid; e < n * n; e += 256) {
+            const int r = j + 1 + e / n, c = j + 1 + e % n;
+            if (r <= c) A[r][c] -= A[j][r] * A[j][c];
+        }
+        __syncthreads();
+    }
+    // in-place inverse of the upper triangle, one column at a time
+    for (int j = 0; j < k; j++) {
+        if (tid < j) {
+            double s = 0;
+            for (int l = tid; l < j; l++) s += A[tid][l] * A[l][j];
+            col[tid] = s;
+        }
+        __syncthreads();
+        const double djj = A[j][j];
+        if (tid < j) A[tid][j] = -col[tid] / djj;
+        __syncthreads();
+        if (tid == 0) A[j][j] = 1.0 / djj;
+        __syncthreads();
+    }
+    for (int e = tid; e < kp * kp; e += 256) {
+        const int i = e / kp, j = e % kp;
+        Rinv[e] = (i < k && j < k && i <= j) ? A[i][j] : 0.0;
+    }
+    if (tid == 0) *status = bad;
+}
+
+hipError_t launch_chol_inv(hipStream_t st, const double *G, int k, int kp, double shift_mult,
+                           int64_t, double *Rinv, int *status)
+{
+    if (kp > kMaxKp || k > kp) return hipErrorInvalidValue;
+    k_chol_inv<<<1, 256, 0, st>>>(G, k, kp, shift_mult, Rinv, status);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ Y = X W ------
+// X: m x kp, W: kp x kp (zero outside the used block), Y: m x kp.  W is
+// staged in LDS; one thread per output element.
+template <int KP>
+__global__ __launch_bounds__(256) void k_apply(const double *__restrict__ X,
+                                               const double *__restrict__ W, int64_t m,
+                                               double *__restrict__ Y)
+{
+    __shared__ double ws[KP][KP];
+    for (int e = threadIdx.x; e < KP * KP; e += 256) ws[e / KP][e % KP] = W[e];
+    __syncthreads();
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= m * KP) return;
+    const int64_t i = idx / KP;
+    const int j = (int)(idx % KP);
+    const double *x = X + i * KP;
+    double s = 0;
+#pragma unroll 16
+    for (int l = 0; l < KP; l++) s += x[l] * ws[l][j];
+    Y[idx] = s;
+}
+
+hipError_t launch_apply(hipStream_t st, const double *X, const double *W, int64_t m, int kp,
+                        double *Y)
+{
+    const unsigned nb = (unsigned)((m * kp + 255) / 256);
+    switch (kp) {
+    case 16: k_apply<16><<<nb, 256, 0, st>>>(X, W, m, Y); break;
+    case 32: k_apply<32><<<nb, 256, 0, st>>>(X, W, m, Y); break;
+    case 48: k_apply<48><<<nb, 256, 0, st>>>(X, W, m, Y); break;
+    case 64: k_apply<64><<<nb, 256, 0, st>>>(X, W, m, Y); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ------------------------------------------------- symmetric eigensolver ---
+// Cyclic Jacobi with the round-robin parallel ordering (Brent & Luk): each
+// round rotates n/2 disjoint pairs at once; n - 1 rounds make a sweep.  The
+// rotation is Golub & Van Loan's sym.schur2.  Input: the upper triangle of B
+// (kp x kp, leading k x k).  Output: eigenvalues in descending order in w[0..k)
+// and the matching unit eigenvectors as the columns of V (kp x kp).
+__global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ B, int k,
+                                                     int kp, double *__restrict__ w,
+                                                     double *__restrict__ V, int *status)
+{
+    __shared__ double A[kMaxKp][kMaxKp + 1];
+    __shared__ double U[kMaxKp][kMaxKp + 1];
+    __shared__ double rc[kMaxKp / 2], rs[kMaxKp / 2];
+    __shared__ int rp[kMaxKp / 2], rq[kMaxKp / 2];
+    __shared__ double red[256];
+    __shared__ int done;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < k * k; e += 256) {
+        const int i = e / k, j = e % k;
+        A[i][j] = (i <= j) ? B[i * kp + j] : B[j * kp + i];
+        U[i][j] = (i == j) ? 1.0 : 0.0;
+    }
+    const int n = (k % 2) ? k + 1 : k;  // one dummy index when k is odd
+    const int half = n / 2;
+    int sweeps = 0;
+    __syncthreads();
+    for (; sweeps < 40; sweeps++) {
+        // convergence: off-diagonal mass against the diagonal
+        double off = 0, dia = 0;
+        for (int e = tid; e < k * k; e += 256) {
+            const int i = e / k, j = e % k;
+            const double v = A[i][j] * A[i][j];
+            if (i == j) dia += v; else off += v;
+        }
+        red[tid] = off;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        const double offs = red[0];
+        __syncthreads();
+        red[tid] = dia;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (tid < o) red[tid] += red[tid + o];
+            __syncthreads();
+        }
+        if (tid == 0) done = !(offs > 1e-34 * red[0]) || offs == 0.0;
+        __syncthreads();
+        if (done) break;
+        for (int round = 0; round < n - 1; round++) {
+            if (tid < half) {
+                int p, q;
+                if (tid == 0) {
+                    p = 0;
+                    q = 1 + round % (n - 1);
+                } else {
+                    p = 1 + (round + tid) % (n - 1);
+                    q = 1 + (round + n - 1 - tid) % (n - 1);
+                }
+                if (p > q) { const int t = p; p = q; q = t; }
+                double c = 1.0, s = 0.0;
+                if (q < k) {
+                    const double apq = A[p][q];
+                    if (apq != 0.0) {
+                        const double tau = (A[q][q] - A[p][p]) / (2.0 * apq);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) /
+                                         (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        s = t * c;
+                    }
+                }
+                rp[tid] = p;
+                rq[tid] = q;
+                rc[tid] = c;
+                rs[tid] = s;
+            }
+            __syncthreads();
+            // rows: A <- J^T A
+            for (int e = tid; e < half * k; e += 256) {
+                const int pi = e / k, j = e % k;
+                const int p = rp[pi], q = rq[pi];
+                if (q < k) {
+                    const double c = rc[pi], s = rs[pi];
+                    const double ap = A[p][j], aq = A[q][j];
+                    A[p][j] = c * ap - s * aq;
+                    A[q][j] = s * ap + c * aq;
+                }
+            }
+            __syncthreads();
+            // columns: A <- A J,  U <- U J
+            for (int e = tid; e < half * k; e += 256) {
+                const int pi = e / k, i = e % k;
+                const int p = rp[pi], q = rq[pi];
+                if (q < k) {
+                    const double c = rc[pi], s = rs[pi];
+                    const double ap = A[i][p], aq = A[i][q];
+                    A[i][p] = c * ap - s * aq;
+                    A[i][q] = s * ap + c * aq;
+                    const double up = U[i][p], uq = U[i][q];
+                    U[i][p] = c * up - s * uq;
+                    U[i][q] = s * up + c * uq;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // descending order: rank of each diagonal entry (ties by index)
+    if (tid < k) {
+        const double d = A[tid][tid];
+        int rank = 0;
+        for (int j = 0; j < k; j++) {
+            const double e = A[j][j];
+            if (e > d || (e == d && j < tid)) rank++;
+        }
+        w[rank] = d;
+        for (int i = 0; i < kp; i++) V[i * kp + rank] = (i < k) ? U[i][tid] : 0.0;
+    }
+    for (int e = tid; e < kp * kp; e += 256) {
+        const int i = e / kp, j = e % kp;
+        if (j >= k) V[e] = 0.0;
+        (void)i;
+    }
+    if (tid == 0) *status = (sweeps >= 40) ? 1 : 0;
+}
+
+hipError_t launch_syev(hipStream_t st, const double *B, int k, int kp, double *w, double *V,
+                       int *status)
+{
+    if (kp > kMaxKp || k > kp) return hipErrorInvalidValue;
+    k_syev_jacobi<<<1, 256, 0, st>>>(B, k, kp, w, V, status);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------- column dot products -----
+// part[p][j] = sum_{i in rows of p} X[i][j] * Y[i][j], j < ncols.
+__global__ __launch_bounds__(256) void k_col_dots(const double *__restrict__ X,
+                                                  const double *__restrict__ Y, int64_t m,
+                                                  int kp, int ncols, int64_t rpp,
+                                                  double *__restrict__ part)
+{
+    __shared__ double red[256];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * rpp;
+    int64_t r1 = r0 + rpp;
+    if (r1 > m) r1 = m;
+    const int rows_per_pass = 256 / kp;
+    const int j = tid % kp, lr = tid / kp;
+    double s = 0;
+    if (lr < rows_per_pass && j < ncols)
+        for (int64_t i = r0 + lr; i < r1; i += rows_per_pass)
+            s += (Y ? X[i * kp + j] * Y[i * kp + j] : X[i * kp + j]);
+    red[tid] = (lr < rows_per_pass) ? s : 0.0;
+    __syncthreads();
+    if (tid < kp) {
+        double t = 0;
+        for (int l = 0; l < rows_per_pass; l++) t += red[l * kp + tid];
+        part[(int64_t)blockIdx.x * kp + tid] = (tid < ncols) ? t : 0.0;
+    }
+}
+
+hipError_t launch_col_dots(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
+                           int ncols, double *part, int nparts, double *dots)
+{
+    int64_t rpp = (m + nparts - 1) / nparts;
+    k_col_dots<<<nparts, 256, 0, st>>>(X, Y, m, kp, ncols, rpp, part);
+    k_sum_parts<<<(kp + 255) / 256, 256, 0, st>>>(part, nparts, kp, dots);
+    return hipGetLastError();
+}
+
+hipError_t launch_colsum(hipStream_t st, const double *X, int64_t m, int kp, double *part,
+                         int nparts, double *out)
+{
+    return launch_col_dots(st, X, nullptr, m, kp, kp, part, nparts, out);
+}
+
+}  // namespace dpca
