@@ -1,0 +1,271 @@
+"""distpca_amd — Python binding of libdpca (include/dpca.h) through ctypes.
+
+The compute path is the HIP library ``distpca_amd/libdpca.so`` built for
+gfx950; this module only marshals arguments.  It raises at import of the
+library when the shared object is missing: there is no CPU fallback.
+
+Mirrors the reference's process interface (DistPCA.exe flags, defaults and
+outputs, src/distpca.cpp:47-644) as ``Params`` / ``PCA``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdpca.so")
+EXE_PATH = os.path.join(_HERE, "DistPCA.exe")
+
+FP64, MIXED = 0, 1
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("nsv", ctypes.c_int), ("nrhs", ctypes.c_int), ("power", ctypes.c_int),
+        ("crit", ctypes.c_int), ("tol", ctypes.c_double), ("bsize", ctypes.c_int),
+        ("miter", ctypes.c_int), ("verbose", ctypes.c_int), ("fwrite", ctypes.c_int),
+        ("fullsvd", ctypes.c_int), ("memory", ctypes.c_double),
+        ("prefix", ctypes.c_char * 1024), ("precision", ctypes.c_int),
+        ("device", ctypes.c_int), ("device_cap", ctypes.c_longlong),
+        ("strict_passes", ctypes.c_int),
+    ]
+
+
+class Report(ctypes.Structure):
+    _fields_ = [
+        ("M", ctypes.c_int), ("N", ctypes.c_longlong), ("nsv", ctypes.c_int),
+        ("nrhs", ctypes.c_int), ("power", ctypes.c_int), ("bsize", ctypes.c_int),
+        ("crit", ctypes.c_int), ("total_its", ctypes.c_int), ("passes", ctypes.c_int),
+        ("incore", ctypes.c_int), ("streamed", ctypes.c_int), ("qr_shifted", ctypes.c_int),
+        ("t_generate_rhs", ctypes.c_double), ("t_load", ctypes.c_double),
+        ("t_mm", ctypes.c_double), ("t_mm_a", ctypes.c_double), ("t_mm_at", ctypes.c_double),
+        ("t_gs", ctypes.c_double), ("t_rr", ctypes.c_double), ("t_other", ctypes.c_double),
+        ("wall", ctypes.c_double), ("last_crit", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class GenSpec(ctypes.Structure):
+    _fields_ = [
+        ("M", ctypes.c_int), ("N", ctypes.c_longlong), ("npop", ctypes.c_int),
+        ("fst", ctypes.c_double), ("admix_frac", ctypes.c_double),
+        ("missing", ctypes.c_double), ("seed", ctypes.c_ulonglong),
+        ("pmin", ctypes.c_double), ("pmax", ctypes.c_double),
+    ]
+
+
+EXPORTS = [
+    "dpca_params_init", "dpca_open", "dpca_open_packed", "dpca_open_synthetic",
+    "dpca_write_synthetic", "dpca_get_dims", "dpca_omega", "dpca_snp_stats", "dpca_decode",
+    "dpca_copy_packed", "dpca_pass", "dpca_run", "dpca_crit_history", "dpca_write_outputs",
+    "dpca_orthonormalize", "dpca_syev", "dpca_close", "dpca_last_error",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libdpca.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"libdpca.so not built ({LIB_PATH}); run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double
+    P = ctypes.POINTER
+    L.dpca_last_error.restype = ctypes.c_char_p
+    L.dpca_params_init.argtypes = [P(_Params)]
+    L.dpca_open.argtypes = [ctypes.c_char_p, P(_Params), P(vp)]
+    L.dpca_open_packed.argtypes = [vp, i32, i64, P(_Params), P(vp)]
+    L.dpca_open_synthetic.argtypes = [P(GenSpec), P(_Params), P(vp)]
+    L.dpca_write_synthetic.argtypes = [ctypes.c_char_p, P(GenSpec)]
+    L.dpca_get_dims.argtypes = [vp, P(i32), P(i64), P(i32), P(i32), P(i32)]
+    L.dpca_omega.argtypes = [i32, i32, vp]
+    L.dpca_snp_stats.argtypes = [vp, vp, vp, vp]
+    L.dpca_decode.argtypes = [vp, i64, i64, vp]
+    L.dpca_copy_packed.argtypes = [vp, i64, i64, vp]
+    L.dpca_pass.argtypes = [vp, vp, vp]
+    L.dpca_run.argtypes = [vp, vp, vp, P(Report)]
+    L.dpca_crit_history.argtypes = [vp, vp, i32]
+    L.dpca_write_outputs.argtypes = [vp, ctypes.c_char_p, vp, vp]
+    L.dpca_orthonormalize.argtypes = [vp, i64, i32, vp, P(i32), i32]
+    L.dpca_syev.argtypes = [vp, i32, vp, vp, i32]
+    L.dpca_close.argtypes = [vp]
+    L.dpca_close.restype = None
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f"libdpca error {rc}: {lib().dpca_last_error().decode()}")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class Params:
+    """DistPCA.exe flags (defaults of src/utilities.cpp:195-232)."""
+    nsv: int = 10
+    nrhs: int = 20
+    power: int = 1
+    crit: int = 1
+    tol: float = 1e-3
+    bsize: int = 0
+    miter: int = 100
+    verbose: int = 1
+    fwrite: int = 0
+    memory: float = 0.0
+    prefix: str = "noprefix"
+    precision: int = FP64
+    device: int = 0
+    device_cap: int = 0
+
+    def c(self) -> _Params:
+        p = _Params()
+        lib().dpca_params_init(ctypes.byref(p))
+        for f in ("nsv", "nrhs", "power", "crit", "tol", "bsize", "miter", "verbose",
+                  "fwrite", "memory", "precision", "device", "device_cap"):
+            setattr(p, f, getattr(self, f))
+        p.prefix = self.prefix.encode()
+        return p
+
+
+def omega(M: int, k: int) -> np.ndarray:
+    """Reference-exact Omega (host side of libdpca)."""
+    out = np.empty((M, k), np.float64)
+    _check(lib().dpca_omega(M, k, _ptr(out)))
+    return out
+
+
+def write_synthetic(prefix: str, spec: GenSpec):
+    _check(lib().dpca_write_synthetic(prefix.encode(), ctypes.byref(spec)))
+
+
+def gen_spec(M, N, npop=3, fst=0.05, admix_frac=0.0, missing=0.01, seed=0,
+             pmin=0.05, pmax=0.5) -> GenSpec:
+    return GenSpec(M, N, npop, fst, admix_frac, missing, seed, pmin, pmax)
+
+
+class PCA:
+    """One libdpca context (dpca_ctx)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        M, N, nsv, nrhs, b = (ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int(),
+                              ctypes.c_int(), ctypes.c_int())
+        _check(lib().dpca_get_dims(self._h, ctypes.byref(M), ctypes.byref(N),
+                                   ctypes.byref(nsv), ctypes.byref(nrhs), ctypes.byref(b)))
+        self.M, self.N, self.nsv, self.nrhs, self.bsize = M.value, N.value, nsv.value, nrhs.value, b.value
+
+    @classmethod
+    def from_bfile(cls, bfile: str, params: Params):
+        h = ctypes.c_void_p()
+        p = params.c()
+        _check(lib().dpca_open(bfile.encode(), ctypes.byref(p), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_packed(cls, packed: np.ndarray, M: int, N: int, params: Params):
+        packed = np.ascontiguousarray(packed, dtype=np.uint8)
+        assert packed.size >= N * ((M + 3) // 4)
+        h = ctypes.c_void_p()
+        p = params.c()
+        _check(lib().dpca_open_packed(_ptr(packed), M, N, ctypes.byref(p), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def synthetic(cls, spec: GenSpec, params: Params):
+        h = ctypes.c_void_p()
+        p = params.c()
+        _check(lib().dpca_open_synthetic(ctypes.byref(spec), ctypes.byref(p), ctypes.byref(h)))
+        return cls(h)
+
+    def close(self):
+        if self._h:
+            lib().dpca_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def snp_stats(self):
+        s = np.empty(self.N, np.int64)
+        c = np.empty(self.N, np.int64)
+        z = np.empty((self.N, 4), np.float64)
+        _check(lib().dpca_snp_stats(self._h, _ptr(s), _ptr(c), _ptr(z)))
+        return s, c, z
+
+    def decode(self, snp0: int, rows: int) -> np.ndarray:
+        out = np.empty((rows, self.M), np.uint8)
+        _check(lib().dpca_decode(self._h, snp0, rows, _ptr(out)))
+        return out
+
+    def packed(self, snp0: int = 0, rows: int | None = None) -> np.ndarray:
+        rows = self.N - snp0 if rows is None else rows
+        out = np.empty((rows, (self.M + 3) // 4), np.uint8)
+        _check(lib().dpca_copy_packed(self._h, snp0, rows, _ptr(out)))
+        return out
+
+    def pass_(self, Q: np.ndarray) -> np.ndarray:
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        assert Q.shape == (self.M, self.nrhs)
+        out = np.empty_like(Q)
+        _check(lib().dpca_pass(self._h, _ptr(Q), _ptr(out)))
+        return out
+
+    def run(self, rhs: np.ndarray | None = None):
+        """Returns (sigma[nsv], rhs[M, nrhs], report dict)."""
+        if rhs is None:
+            rhs = omega(self.M, self.nrhs)
+        rhs = np.array(rhs, dtype=np.float64, order="C", copy=True)
+        assert rhs.shape == (self.M, self.nrhs)
+        sigma = np.empty(self.nsv, np.float64)
+        rep = Report()
+        _check(lib().dpca_run(self._h, _ptr(rhs), _ptr(sigma), ctypes.byref(rep)))
+        return sigma, rhs, rep.as_dict()
+
+    def crit_history(self):
+        buf = np.empty(1024, np.float64)
+        n = lib().dpca_crit_history(self._h, _ptr(buf), 1024)
+        return buf[:n].copy()
+
+    def write_outputs(self, prefix: str, rhs: np.ndarray, sigma: np.ndarray):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        _check(lib().dpca_write_outputs(self._h, prefix.encode(), _ptr(rhs), _ptr(sigma)))
+
+
+def orthonormalize(A: np.ndarray, device: int = 0):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    m, k = A.shape
+    Q = np.empty_like(A)
+    sh = ctypes.c_int()
+    _check(lib().dpca_orthonormalize(_ptr(A), m, k, _ptr(Q), ctypes.byref(sh), device))
+    return Q, bool(sh.value)
+
+
+def syev(B: np.ndarray, device: int = 0):
+    B = np.ascontiguousarray(B, dtype=np.float64)
+    k = B.shape[0]
+    w = np.empty(k, np.float64)
+    X = np.empty((k, k), np.float64)
+    _check(lib().dpca_syev(_ptr(B), k, _ptr(w), _ptr(X), device))
+    return w, X

@@ -1,0 +1,1127 @@
+// Host side of libdpca: context, PLINK input, Omega, the subspace-iteration
+// driver and the C ABI of include/dpca.h.
+//
+// The driver mirrors BlockSubspaceIter_MPI_OOC_double_buffering
+// (src/methods.cpp:702-1135) for one rank, with every per-pass and
+// per-iteration step on the device: the fused decode + product kernels for
+// A A^T Q, CholQR2 instead of dgeqrf/dorgqr, a device Jacobi eigensolver
+// instead of dsyev, and the stopping test.  The host only issues launches and
+// reads a few bytes of criterion state once per outer iteration.
+#include <errno.h>
+#include <fcntl.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dpca.h"
+#include "dpca_gen.h"
+#include "dpca_internal.h"
+
+namespace dpca {
+
+static thread_local char g_err[2048] = "";
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+static double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+static int host_threads()
+{
+    unsigned h = std::thread::hardware_concurrency();
+    if (h == 0) h = 4;
+    return h > 16 ? 16 : (int)h;
+}
+
+// ------------------------------------------------------------------ Omega --
+// Park-Miller minimal standard generator with Schrage's factorisation and the
+// one-sided Box-Muller transform of src/gennorm.c:21-72, driven in the order
+// of src/distpca.cpp:389-394.  A positive seed restarts the state at the seed;
+// a non-positive one (the reference's int seed jj*M + kk wraps for
+// M >= 46341) advances it one step, so elements are independent from each
+// positive seed on: the matrix is cut into chunks that start at positive
+// seeds and the chunks are generated in parallel.
+namespace omega {
+struct Lcg {
+    long x = 0;
+    double next()
+    {
+        const long q = 127773, r = 2836, a = 16807, m = 2147483647;
+        const long hi = x / q, lo = x % q;
+        const long t = a * lo - r * hi;
+        x = (t > 0) ? t : t + m;
+        return (double)x / m;
+    }
+};
+
+static inline int32_t seed_of(int64_t e, int M, int k)
+{
+    const uint32_t jj = (uint32_t)(e / k), kk = (uint32_t)(e % k);
+    return (int32_t)(jj * (uint32_t)M + kk);
+}
+
+static void gen_range(int64_t e0, int64_t e1, int M, int k, long state, double *out)
+{
+#pragma clang fp contract(off)
+    Lcg g;
+    g.x = state;
+    const double pi = 3.14159265;
+    for (int64_t e = e0; e < e1; e++) {
+        const int32_t s = seed_of(e, M, k);
+        if (s > 0)
+            g.x = s;
+        else
+            g.next();
+        double u = 0.0;
+        while (u == 0.0) u = g.next();
+        const double rr = sqrt(-2.0 * log(u));
+        double th = 0.0;
+        while (th == 0.0) th = 2.0 * pi * g.next();
+        const double x = rr * cos(th);
+        out[e] = (x * 1.0) + 0.0;
+    }
+}
+}  // namespace omega
+
+int omega_generate(int M, int k, double *out)
+{
+    const int64_t n = (int64_t)M * k;
+    const int T = host_threads();
+    if (n < 65536 || T == 1) {
+        omega::gen_range(0, n, M, k, 0, out);
+        return 0;
+    }
+    // chunk starts: first element at/after an even split whose seed is > 0
+    std::vector<int64_t> starts{0};
+    for (int t = 1; t < T; t++) {
+        int64_t e = n * t / T;
+        while (e < n && omega::seed_of(e, M, k) <= 0) e++;
+        if (e > starts.back() && e < n) starts.push_back(e);
+    }
+    starts.push_back(n);
+    std::vector<std::thread> th;
+    for (size_t c = 0; c + 1 < starts.size(); c++)
+        th.emplace_back([=] { omega::gen_range(starts[c], starts[c + 1], M, k, 0, out); });
+    for (auto &x : th) x.join();
+    // chunk 0 starts at element 0 with the program-start state 0, exactly as
+    // the reference; every later chunk starts at a positive seed, so the
+    // state it was given is overwritten before use.
+    return 0;
+}
+
+}  // namespace dpca
+
+using namespace dpca;
+
+// ----------------------------------------------------------------- context --
+struct Timer {
+    int kind;
+    hipEvent_t a, b;
+};
+enum { T_LOAD, T_MM_A, T_MM_AT, T_MM, T_GS, T_RR, T_OTHER, T_NKIND };
+
+struct dpca_ctx {
+    dpca_params p{};
+    int dev = 0;
+    hipStream_t st = nullptr;
+    int M = 0;
+    int64_t N = 0;
+    int k = 0, kp = 0, nsv = 0, bsize = 0;
+    int64_t np = 0, ldp = 0, Mpad = 0, Npad = 0;
+    bool incore = false;
+    std::vector<std::string> iids;
+    // device buffers
+    uint8_t *packed = nullptr;        // Npad x ldp
+    double *zval = nullptr, *mu = nullptr, *dsc = nullptr;
+    int64_t *ssum = nullptr, *sctr = nullptr;
+    double *Q = nullptr, *ARHS = nullptr, *RHS2 = nullptr, *RHS2old = nullptr, *tmp = nullptr;
+    double *T = nullptr;              // Npad x kp
+    double *part = nullptr;           // splits x Mpad x kp
+    int splits = 1;
+    double *gpart = nullptr, *G = nullptr, *Rinv = nullptr, *w = nullptr, *X = nullptr,
+           *dots = nullptr;
+    int ngparts = 256;
+    int *flags = nullptr;             // [0] shifted used, [1] QR failure, [2] Jacobi
+    IterState *its = nullptr;
+    std::vector<double> crit_hist;
+    // timers
+    std::vector<Timer> pending;
+    std::vector<hipEvent_t> free_ev;
+    double t[T_NKIND] = {};
+    double t_generate_rhs = 0;
+    int passes = 0;
+};
+
+static hipEvent_t get_event(dpca_ctx *c)
+{
+    if (!c->free_ev.empty()) {
+        hipEvent_t e = c->free_ev.back();
+        c->free_ev.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct Stage {
+    dpca_ctx *c;
+    Timer tm;
+    Stage(dpca_ctx *ctx, int kind) : c(ctx)
+    {
+        tm.kind = kind;
+        tm.a = get_event(c);
+        tm.b = get_event(c);
+        if (tm.a) (void)hipEventRecord(tm.a, c->st);
+    }
+    ~Stage()
+    {
+        if (tm.b) (void)hipEventRecord(tm.b, c->st);
+        c->pending.push_back(tm);
+    }
+};
+
+// Call after a stream synchronisation: fold the recorded stage times in.
+static void collect_timers(dpca_ctx *c)
+{
+    for (auto &tm : c->pending) {
+        float ms = 0;
+        if (tm.a && tm.b && hipEventElapsedTime(&ms, tm.a, tm.b) == hipSuccess)
+            c->t[tm.kind] += ms * 1e-3;
+        if (tm.a) c->free_ev.push_back(tm.a);
+        if (tm.b) c->free_ev.push_back(tm.b);
+    }
+    c->pending.clear();
+}
+
+template <class P>
+static int dalloc(P **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+        return -3;
+    }
+    return 0;
+}
+
+static void free_ctx(dpca_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->dev);
+    void *ptrs[] = {c->packed, c->zval, c->mu, c->dsc, c->ssum, c->sctr, c->Q, c->ARHS,
+                    c->RHS2, c->RHS2old, c->tmp, c->T, c->part, c->gpart, c->G, c->Rinv,
+                    c->w, c->X, c->dots, c->flags, c->its};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (auto &tm : c->pending) {
+        if (tm.a) (void)hipEventDestroy(tm.a);
+        if (tm.b) (void)hipEventDestroy(tm.b);
+    }
+    for (auto e : c->free_ev) hipEventDestroy(e);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+}
+
+// bsize and NSV/NRHS clamps of src/distpca.cpp:218-352 for a single rank.
+static int resolve_params(dpca_ctx *c, const dpca_params *in)
+{
+    c->p = *in;
+    dpca_params &p = c->p;
+    const int M = c->M;
+    const int64_t N = c->N;
+    if (M <= 0 || N <= 0) {
+        set_error("M and/or N were either zero or negative. Aborting...");
+        return -1;
+    }
+    if (N > 0x7fffffffLL) {
+        set_error("N = %lld exceeds the reference's int range", (long long)N);
+        return -1;
+    }
+    if (p.bsize <= 0) {
+        long pages = sysconf(_SC_PHYS_PAGES), psz = sysconf(_SC_PAGE_SIZE);
+        const double ram_gb = (double)pages * (double)psz / 1024.0 / 1e6;  // KB / 1e6
+        const double mem = (p.memory == 0.0) ? ram_gb / 10.0 : p.memory;
+        const double membuff = 3.0 * N * 8.0 + (double)M * p.nsv * 8.0 + 2.0 * N * p.nrhs +
+                               3.0 * M * 8.0 + 2048.0 * 100000.0;
+        const double avail = mem * 1e9 - membuff;
+        const double blk = avail > 0.0 ? avail / (8.0 * M) : (membuff + M * 8.0) / 1e7;
+        p.bsize = (int)blk;
+        if (p.bsize <= 0) p.bsize = p.nsv;
+        if (p.bsize >= N) p.bsize = (int)N;
+    } else {
+        if (p.bsize > N) p.bsize = (int)N;
+        if (p.bsize < 1) p.bsize = p.nsv;
+    }
+    if (p.nsv <= 0) {
+        set_error("NSV was either zero or negative. Aborting...");
+        return -1;
+    }
+    const int mn = (int)std::min<int64_t>(M, N);
+    if (p.nsv > mn) p.nsv = mn;
+    if (p.nrhs <= 0) p.nrhs = std::min(2 * p.nsv, mn);
+    if (p.nrhs > mn) p.nrhs = mn;
+    if (p.nrhs < p.nsv) p.nrhs = std::min(2 * p.nsv, mn);
+    if (p.nrhs > 64) {
+        set_error("nrhs = %d: this build supports at most 64 right-hand sides", p.nrhs);
+        return -1;
+    }
+    if (p.crit < 0 || p.crit > 2) {
+        set_error("crit must be 0, 1 or 2");
+        return -1;
+    }
+    if (p.power < 1) p.power = 1;  // the reference always runs at least one pass pair
+    c->k = p.nrhs;
+    c->kp = (int)round_up(c->k, 16);
+    c->nsv = p.nsv;
+    c->bsize = p.bsize;
+    c->incore = (p.bsize == N);
+    c->np = (M + 3) / 4;
+    c->ldp = round_up(c->np, kLdpAlign);
+    c->Mpad = 4 * c->ldp;
+    c->Npad = round_up(N, kRowAlign);
+    return 0;
+}
+
+static int alloc_work(dpca_ctx *c)
+{
+    const size_t mk = (size_t)c->Mpad * c->kp * sizeof(double);
+    int r = 0;
+    r |= dalloc(&c->zval, (size_t)c->Npad * 4 * sizeof(double));
+    r |= dalloc(&c->mu, (size_t)c->Npad * sizeof(double));
+    r |= dalloc(&c->dsc, (size_t)c->Npad * sizeof(double));
+    r |= dalloc(&c->ssum, (size_t)c->Npad * sizeof(int64_t));
+    r |= dalloc(&c->sctr, (size_t)c->Npad * sizeof(int64_t));
+    r |= dalloc(&c->Q, mk);
+    r |= dalloc(&c->ARHS, mk);
+    r |= dalloc(&c->RHS2, mk);
+    r |= dalloc(&c->RHS2old, mk);
+    r |= dalloc(&c->tmp, mk);
+    r |= dalloc(&c->T, (size_t)c->Npad * c->kp * sizeof(double));
+    // K_B splits: aim for >= ~2048 workgroups, each split >= 64 rows
+    const int64_t tiles = c->ldp / 64;
+    int64_t sp = (2048 + tiles - 1) / tiles;
+    sp = std::max<int64_t>(1, std::min<int64_t>(sp, c->Npad / 256));
+    c->splits = (int)sp;
+    r |= dalloc(&c->part, (size_t)c->splits * mk);
+    r |= dalloc(&c->gpart, (size_t)c->ngparts * 64 * 64 * sizeof(double));
+    r |= dalloc(&c->G, 64 * 64 * sizeof(double));
+    r |= dalloc(&c->Rinv, 64 * 64 * sizeof(double));
+    r |= dalloc(&c->w, 64 * sizeof(double));
+    r |= dalloc(&c->X, 64 * 64 * sizeof(double));
+    r |= dalloc(&c->dots, 64 * sizeof(double));
+    r |= dalloc(&c->flags, 16 * sizeof(int));
+    r |= dalloc(&c->its, sizeof(IterState));
+    if (r) return -3;
+    DPCA_CHECK(hipMemsetAsync(c->Q, 0, mk, c->st));
+    DPCA_CHECK(hipMemsetAsync(c->ARHS, 0, mk, c->st));
+    DPCA_CHECK(hipMemsetAsync(c->RHS2, 0, mk, c->st));
+    DPCA_CHECK(hipMemsetAsync(c->RHS2old, 0, mk, c->st));
+    DPCA_CHECK(hipMemsetAsync(c->tmp, 0, mk, c->st));
+    return 0;
+}
+
+static int compute_stats(dpca_ctx *c)
+{
+    Stage s(c, T_LOAD);
+    DPCA_CHECK(launch_snp_stats(c->st, c->packed, c->Npad, c->ldp, 1.0 / sqrt((double)c->N),
+                                c->ssum, c->sctr, c->zval, c->mu, c->dsc));
+    return 0;
+}
+
+static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out)
+{
+    *out = nullptr;
+    if (!p) {
+        set_error("params is NULL");
+        return -1;
+    }
+    dpca_ctx *c = new dpca_ctx();
+    c->dev = p->device;
+    c->M = M;
+    c->N = N;
+    hipError_t e = hipSetDevice(c->dev);
+    if (e != hipSuccess) {
+        set_error("hipSetDevice(%d): %s", c->dev, hipGetErrorString(e));
+        delete c;
+        return -2;
+    }
+    int r = resolve_params(c, p);
+    if (r) {
+        delete c;
+        return r;
+    }
+    if (c->p.precision != DPCA_FP64) {
+        set_error("precision %d is not available in this build", c->p.precision);
+        delete c;
+        return -1;
+    }
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        delete c;
+        return -2;
+    }
+    if ((r = alloc_work(c))) {
+        free_ctx(c);
+        return r;
+    }
+    if ((r = dalloc(&c->packed, (size_t)c->Npad * c->ldp))) {
+        free_ctx(c);
+        return r;
+    }
+    *out = c;
+    return 0;
+}
+
+// Raw .bed rows from host memory -> padded device layout, in chunks.
+static int upload_rows(dpca_ctx *c, const uint8_t *host, int64_t rows)
+{
+    Stage s(c, T_LOAD);
+    const int64_t chunk_rows = std::max<int64_t>(1, (256ll << 20) / c->np);
+    uint8_t *raw = nullptr;
+    const int64_t nrows = std::min(chunk_rows, rows);
+    if (dalloc(&raw, (size_t)nrows * c->np)) return -3;
+    int rc = 0;
+    for (int64_t r0 = 0; r0 < rows && !rc; r0 += chunk_rows) {
+        const int64_t n = std::min(chunk_rows, rows - r0);
+        hipError_t e = hipMemcpyAsync(raw, host + r0 * c->np, (size_t)(n * c->np),
+                                      hipMemcpyHostToDevice, c->st);
+        if (e == hipSuccess)
+            e = launch_repack(c->st, raw, n, c->np, c->packed + r0 * c->ldp, c->ldp, c->M);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+        if (e != hipSuccess) {
+            set_error("upload: %s", hipGetErrorString(e));
+            rc = -2;
+        }
+    }
+    (void)hipFree(raw);
+    if (rc) return rc;
+    DPCA_CHECK(launch_fill_u8(c->st, c->packed + rows * c->ldp, (c->Npad - rows) * c->ldp,
+                              kPadByte));
+    return 0;
+}
+
+// --------------------------------------------------------------- one pass --
+// ARHS = Z^T (Z Qin) over the resident matrix (src/methods.cpp:786-936).
+static int pass_fp64(dpca_ctx *c, const double *Qin, double *out)
+{
+    {
+        Stage s(c, T_MM_A);
+        DPCA_CHECK(launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, Qin, c->kp, c->T));
+    }
+    {
+        Stage s(c, T_MM_AT);
+        DPCA_CHECK(launch_zt_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->T, c->kp,
+                                 c->splits, c->part));
+        DPCA_CHECK(launch_reduce_parts(c->st, c->part, c->splits, c->Mpad * c->kp, out, 0));
+    }
+    c->passes++;
+    return 0;
+}
+
+static int do_pass(dpca_ctx *c, const double *Qin, double *out) { return pass_fp64(c, Qin, out); }
+
+// CholQR2 (+ shifted fallback) of A (Mpad x kp) into c->Q; uses c->tmp.
+static int orthonormalise(dpca_ctx *c, const double *A)
+{
+    Stage s(c, T_GS);
+    const double u = ldexp(1.0, -53);
+    const double shift = 11.0 * ((double)c->M * c->k + (double)c->k * (c->k + 1)) * u;
+    const double *src = A;
+    double *dst[3] = {c->tmp, c->Q, c->tmp};
+    for (int mode = 0; mode < 3; mode++) {
+        DPCA_CHECK(launch_gram(c->st, src, src, c->Mpad, c->kp, c->gpart, c->ngparts, c->G));
+        DPCA_CHECK(launch_chol_inv(c->st, c->G, c->k, c->kp, shift, mode, c->Rinv, c->flags));
+        DPCA_CHECK(launch_apply(c->st, src, c->Rinv, c->Mpad, c->kp, dst[mode]));
+        src = dst[mode];
+    }
+    std::swap(c->Q, c->tmp);  // the third pass landed in tmp
+    return 0;
+}
+
+static int copy_in_rhs(dpca_ctx *c, const double *host, double *dev)
+{
+    DPCA_CHECK(hipMemsetAsync(dev, 0, (size_t)c->Mpad * c->kp * sizeof(double), c->st));
+    DPCA_CHECK(hipMemcpy2DAsync(dev, c->kp * sizeof(double), host, c->k * sizeof(double),
+                                c->k * sizeof(double), c->M, hipMemcpyHostToDevice, c->st));
+    return 0;
+}
+
+static int copy_out_rhs(dpca_ctx *c, const double *dev, double *host)
+{
+    DPCA_CHECK(hipMemcpy2DAsync(host, c->k * sizeof(double), dev, c->kp * sizeof(double),
+                                c->k * sizeof(double), c->M, hipMemcpyDeviceToHost, c->st));
+    DPCA_CHECK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
+{
+    const double t0 = now_s();
+    const dpca_params &p = c->p;
+    for (int i = 0; i < T_NKIND; i++) c->t[i] = 0;
+    c->passes = 0;
+    c->crit_hist.clear();
+    std::vector<double> om;
+    double *rhs_in = rhs;
+    if (!rhs_in) {
+        const double tg = now_s();
+        om.resize((size_t)c->M * c->k);
+        omega_generate(c->M, c->k, om.data());
+        c->t_generate_rhs = now_s() - tg;
+        rhs_in = om.data();
+    }
+    int r;
+    if ((r = copy_in_rhs(c, rhs_in, c->RHS2))) return r;
+    if (p.crit == 2)
+        DPCA_CHECK(hipMemcpyAsync(c->RHS2old, c->RHS2, (size_t)c->Mpad * c->kp * 8,
+                                  hipMemcpyDeviceToDevice, c->st));
+    DPCA_CHECK(hipMemsetAsync(c->flags, 0, 16 * sizeof(int), c->st));
+    DPCA_CHECK(hipMemsetAsync(c->its, 0, sizeof(IterState), c->st));
+    IterState hs{};
+    int ii = 0;
+    for (ii = 0; ii < p.miter; ii++) {
+        if (c->incore) {
+            // SubspaceIteration_MPI (src/methods.cpp:52-192): `power` full
+            // passes with QR between them, then QR, the half pass Z Q and the
+            // Rayleigh-Ritz matrix (Z Q)^T (Z Q).
+            if ((r = do_pass(c, c->RHS2, c->ARHS))) return r;
+            for (int kk = 0; kk < p.power - 1; kk++) {
+                if ((r = orthonormalise(c, c->ARHS))) return r;
+                if ((r = do_pass(c, c->Q, c->ARHS))) return r;
+            }
+            if ((r = orthonormalise(c, c->ARHS))) return r;
+            {
+                Stage s(c, T_MM_A);
+                DPCA_CHECK(launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->Q, c->kp,
+                                         c->T));
+            }
+            {
+                Stage s(c, T_RR);
+                DPCA_CHECK(launch_gram(c->st, c->T, c->T, c->Npad, c->kp, c->gpart, c->ngparts,
+                                       c->G));
+            }
+        } else {
+            if ((r = do_pass(c, c->RHS2, c->ARHS))) return r;
+            for (int kk = 0; kk < p.power - 1; kk++) {
+                if ((r = orthonormalise(c, c->ARHS))) return r;
+                if ((r = do_pass(c, c->Q, c->ARHS))) return r;
+            }
+            if ((r = orthonormalise(c, c->ARHS))) return r;
+            if ((r = do_pass(c, c->Q, c->ARHS))) return r;
+            Stage s(c, T_RR);
+            DPCA_CHECK(launch_gram(c->st, c->Q, c->ARHS, c->Mpad, c->kp, c->gpart, c->ngparts,
+                                   c->G));
+        }
+        {
+            Stage s(c, T_RR);
+            DPCA_CHECK(launch_syev(c->st, c->G, c->k, c->kp, c->w, c->X, c->flags + 2));
+            DPCA_CHECK(launch_apply(c->st, c->Q, c->X, c->Mpad, c->kp, c->RHS2));
+            if (p.crit == 2 && ii > 0)
+                DPCA_CHECK(launch_col_dots(c->st, c->RHS2old, c->RHS2, c->Mpad, c->kp, c->nsv,
+                                           c->gpart, c->ngparts, c->dots));
+            DPCA_CHECK(launch_crit(c->st, c->w, c->nsv, p.crit, p.tol, ii, c->dots, c->its));
+            DPCA_CHECK(hipMemcpyAsync(&hs, c->its, sizeof(IterState), hipMemcpyDeviceToHost,
+                                      c->st));
+        }
+        DPCA_CHECK(hipStreamSynchronize(c->st));
+        collect_timers(c);
+        if (ii > 0) {
+            c->crit_hist.push_back(hs.crit);
+            if (p.verbose > 1) {
+                if (p.crit == 0)
+                    printf("Iteration %d: rel. error: %02.13f\n", ii, hs.crit);
+                else if (p.crit == 1)
+                    printf("Iteration %d: %d converged\n", ii, hs.conv);
+                else
+                    printf("Iteration %d: MEV error: %02.13f\n", ii, hs.crit);
+            }
+        }
+        if (hs.stop) break;
+        if (p.crit == 2)
+            DPCA_CHECK(hipMemcpyAsync(c->RHS2old, c->RHS2, (size_t)c->Mpad * c->kp * 8,
+                                      hipMemcpyDeviceToDevice, c->st));
+    }
+    const int total_its = (ii < p.miter) ? ii + 1 : ii;
+    int flags[16];
+    DPCA_CHECK(hipMemcpyAsync(flags, c->flags, sizeof flags, hipMemcpyDeviceToHost, c->st));
+    if (rhs && (r = copy_out_rhs(c, c->RHS2, rhs))) return r;
+    DPCA_CHECK(hipStreamSynchronize(c->st));
+    collect_timers(c);
+    if (flags[1]) {
+        set_error("CholeskyQR failed even with the shifted fallback (rank-deficient basis)");
+        return -4;
+    }
+    if (flags[2]) {
+        set_error("Jacobi eigensolver did not converge");
+        return -4;
+    }
+    if (sigma)
+        for (int j = 0; j < c->nsv; j++) sigma[j] = hs.sigma[j];
+    if (rep) {
+        memset(rep, 0, sizeof *rep);
+        rep->M = c->M;
+        rep->N = c->N;
+        rep->nsv = c->nsv;
+        rep->nrhs = c->k;
+        rep->power = p.power;
+        rep->bsize = c->bsize;
+        rep->crit = p.crit;
+        rep->total_its = total_its;
+        rep->passes = c->passes;
+        rep->incore = c->incore;
+        rep->qr_shifted = flags[0];
+        rep->t_generate_rhs = c->t_generate_rhs;
+        rep->t_load = c->t[T_LOAD];
+        rep->t_mm_a = c->t[T_MM_A];
+        rep->t_mm_at = c->t[T_MM_AT];
+        rep->t_mm = c->t[T_MM_A] + c->t[T_MM_AT] + c->t[T_MM];
+        rep->t_gs = c->t[T_GS];
+        rep->t_rr = c->t[T_RR];
+        rep->t_other = c->t[T_OTHER];
+        rep->wall = now_s() - t0;
+        rep->last_crit = hs.crit;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- PLINK IO --
+// Lines of a text file the way src/distpca.cpp:137-147 counts them: every
+// newline-terminated line; an unterminated last line is dropped.
+static int read_lines(const std::string &path, std::vector<std::string> &out)
+{
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) {
+        set_error("Error reading file '%s': %s", path.c_str(), strerror(errno));
+        return -1;
+    }
+    std::string cur;
+    char buf[1 << 16];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) {
+        for (size_t i = 0; i < n; i++) {
+            if (buf[i] == '\n') {
+                out.push_back(cur);
+                cur.clear();
+            } else {
+                cur.push_back(buf[i]);
+            }
+        }
+    }
+    fclose(f);
+    return 0;
+}
+
+static std::string token(const std::string &line, int idx)
+{
+    int t = 0;
+    size_t i = 0;
+    while (i < line.size()) {
+        while (i < line.size() && isspace((unsigned char)line[i])) i++;
+        size_t j = i;
+        while (j < line.size() && !isspace((unsigned char)line[j])) j++;
+        if (j > i) {
+            if (t == idx) return line.substr(i, j - i);
+            t++;
+        }
+        i = j;
+    }
+    return std::string();
+}
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+const char *dpca_last_error(void) { return g_err; }
+
+int dpca_params_init(dpca_params *p)
+{
+    if (!p) return -1;
+    memset(p, 0, sizeof *p);
+    p->nsv = 10;
+    p->nrhs = 20;
+    p->power = 1;
+    p->crit = 1;
+    p->tol = 1e-3;
+    p->bsize = 0;
+    p->miter = 100;
+    p->verbose = 1;
+    p->fwrite = 0;
+    p->fullsvd = 0;
+    p->memory = 0;
+    snprintf(p->prefix, sizeof p->prefix, "noprefix");
+    p->precision = DPCA_FP64;
+    p->device = 0;
+    p->device_cap = 0;
+    p->strict_passes = 1;
+    return 0;
+}
+
+int dpca_omega(int M, int k, double *out)
+{
+    if (M <= 0 || k <= 0 || !out) {
+        set_error("dpca_omega: bad arguments");
+        return -1;
+    }
+    return omega_generate(M, k, out);
+}
+
+int dpca_open(const char *bfile, const dpca_params *p, dpca_ctx **out)
+{
+    if (!bfile || !out) {
+        set_error("dpca_open: NULL argument");
+        return -1;
+    }
+    const std::string base(bfile);
+    const std::string bed = base + ".bed";
+    FILE *fb = fopen(bed.c_str(), "rb");
+    if (!fb) {
+        set_error("[Data::read_bed] Error reading file %s", bed.c_str());
+        return -1;
+    }
+    std::vector<std::string> fam, bim;
+    if (read_lines(base + ".fam", fam) || read_lines(base + ".bim", bim)) {
+        fclose(fb);
+        return -1;
+    }
+    const int M = (int)fam.size();
+    const int64_t N = (int64_t)bim.size();
+    const int64_t np = (M + 3) / 4;
+    struct stat sb;
+    if (fstat(fileno(fb), &sb) != 0 || (int64_t)sb.st_size < 3 + N * np) {
+        set_error("%s holds %lld bytes; %d samples x %lld SNPs need %lld", bed.c_str(),
+                  (long long)sb.st_size, M, (long long)N, (long long)(3 + N * np));
+        fclose(fb);
+        return -1;
+    }
+    dpca_ctx *c = nullptr;
+    int r = new_ctx(p, M, N, &c);
+    if (r) {
+        fclose(fb);
+        return r;
+    }
+    c->iids.reserve(M);
+    for (auto &l : fam) c->iids.push_back(token(l, 1));
+    // read in pinned chunks and stage
+    const int64_t chunk_rows = std::max<int64_t>(1, (256ll << 20) / np);
+    uint8_t *pin = nullptr;
+    if (hipHostMalloc((void **)&pin, (size_t)std::min(chunk_rows, N) * np) != hipSuccess) {
+        set_error("hipHostMalloc failed");
+        fclose(fb);
+        free_ctx(c);
+        return -3;
+    }
+    uint8_t *raw = nullptr;
+    if (dalloc(&raw, (size_t)std::min(chunk_rows, N) * np)) {
+        (void)hipHostFree(pin);
+        fclose(fb);
+        free_ctx(c);
+        return -3;
+    }
+    fseek(fb, 3, SEEK_SET);  // 3-byte magic, not validated (src/distpca.cpp:105)
+    {
+        Stage s(c, T_LOAD);
+        for (int64_t r0 = 0; r0 < N && !r; r0 += chunk_rows) {
+            const int64_t n = std::min(chunk_rows, N - r0);
+            if (fread(pin, 1, (size_t)(n * np), fb) != (size_t)(n * np)) {
+                set_error("short read on %s", bed.c_str());
+                r = -1;
+                break;
+            }
+            hipError_t e = hipMemcpyAsync(raw, pin, (size_t)(n * np), hipMemcpyHostToDevice, c->st);
+            if (e == hipSuccess)
+                e = launch_repack(c->st, raw, n, np, c->packed + r0 * c->ldp, c->ldp, M);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+            if (e != hipSuccess) {
+                set_error("bed upload: %s", hipGetErrorString(e));
+                r = -2;
+            }
+        }
+    }
+    fclose(fb);
+    (void)hipHostFree(pin);
+    (void)hipFree(raw);
+    if (!r && launch_fill_u8(c->st, c->packed + N * c->ldp, (c->Npad - N) * c->ldp, kPadByte) !=
+                  hipSuccess) {
+        set_error("fill failed");
+        r = -2;
+    }
+    if (!r) r = compute_stats(c);
+    if (!r && hipStreamSynchronize(c->st) != hipSuccess) r = -2;
+    if (r) {
+        free_ctx(c);
+        return r;
+    }
+    *out = c;
+    return 0;
+}
+
+int dpca_open_packed(const uint8_t *packed, int M, long long N, const dpca_params *p,
+                     dpca_ctx **out)
+{
+    if (!packed || !out) {
+        set_error("dpca_open_packed: NULL argument");
+        return -1;
+    }
+    dpca_ctx *c = nullptr;
+    int r = new_ctx(p, M, N, &c);
+    if (r) return r;
+    for (int i = 0; i < M; i++) c->iids.push_back(std::to_string(i));
+    if ((r = upload_rows(c, packed, N)) || (r = compute_stats(c))) {
+        free_ctx(c);
+        return r;
+    }
+    if (hipStreamSynchronize(c->st) != hipSuccess) {
+        set_error("stream sync failed");
+        free_ctx(c);
+        return -2;
+    }
+    *out = c;
+    return 0;
+}
+
+static int prep_gen(const dpca_gen_spec *g, GenSpec *spec, GenTables *tab)
+{
+    if (!g) return -1;
+    int r = gen_prepare(g->M, g->N, g->npop, g->fst, g->admix_frac, g->missing, g->seed,
+                        g->pmin > 0 ? g->pmin : 0.05, g->pmax > 0 ? g->pmax : 0.5, spec, tab);
+    if (r) set_error("invalid generator spec");
+    return r;
+}
+
+int dpca_open_synthetic(const dpca_gen_spec *g, const dpca_params *p, dpca_ctx **out)
+{
+    GenSpec spec;
+    GenTables tab;
+    if (prep_gen(g, &spec, &tab)) return -1;
+    dpca_ctx *c = nullptr;
+    int r = new_ctx(p, g->M, g->N, &c);
+    if (r) {
+        gen_free(&tab);
+        return r;
+    }
+    for (int i = 0; i < g->M; i++) c->iids.push_back(std::to_string(i));
+    uint32_t *dthr = nullptr;
+    uint16_t *dwts = nullptr;
+    const size_t nthr = (size_t)g->N * g->npop, nw = (size_t)(g->M - spec.m_pure) * g->npop;
+    if (dalloc(&dthr, nthr * 4) || dalloc(&dwts, std::max<size_t>(nw, 1) * 2)) {
+        gen_free(&tab);
+        free_ctx(c);
+        return -3;
+    }
+    hipError_t e = hipMemcpy(dthr, tab.thr, nthr * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nw) e = hipMemcpy(dwts, tab.wts, nw * 2, hipMemcpyHostToDevice);
+    {
+        Stage s(c, T_LOAD);
+        if (e == hipSuccess)
+            e = launch_generate(c->st, spec, dthr, dwts, 0, c->Npad, c->packed, c->ldp);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(dthr);
+    (void)hipFree(dwts);
+    gen_free(&tab);
+    if (e != hipSuccess) {
+        set_error("generate: %s", hipGetErrorString(e));
+        free_ctx(c);
+        return -2;
+    }
+    if ((r = compute_stats(c)) || hipStreamSynchronize(c->st) != hipSuccess) {
+        free_ctx(c);
+        return r ? r : -2;
+    }
+    collect_timers(c);
+    *out = c;
+    return 0;
+}
+
+int dpca_write_synthetic(const char *prefix, const dpca_gen_spec *g)
+{
+    GenSpec spec;
+    GenTables tab;
+    if (!prefix || prep_gen(g, &spec, &tab)) return -1;
+    const std::string base(prefix);
+    FILE *f = fopen((base + ".fam").c_str(), "w");
+    if (!f) {
+        set_error("cannot write %s.fam", prefix);
+        gen_free(&tab);
+        return -1;
+    }
+    for (int i = 0; i < g->M; i++) {
+        const int pop = i < spec.m_pure ? gen_pop(spec, i) : -1;
+        if (pop >= 0)
+            fprintf(f, "POP%d IND%d 0 0 0 -9\n", pop, i);
+        else
+            fprintf(f, "ADMIX IND%d 0 0 0 -9\n", i);
+    }
+    fclose(f);
+    f = fopen((base + ".bim").c_str(), "w");
+    if (!f) {
+        set_error("cannot write %s.bim", prefix);
+        gen_free(&tab);
+        return -1;
+    }
+    for (int64_t s = 0; s < g->N; s++)
+        fprintf(f, "1\tsnp%lld\t0\t%lld\tA\tG\n", (long long)s, (long long)(s + 1));
+    fclose(f);
+    f = fopen((base + ".bed").c_str(), "wb");
+    if (!f) {
+        set_error("cannot write %s.bed", prefix);
+        gen_free(&tab);
+        return -1;
+    }
+    const uint8_t magic[3] = {0x6c, 0x1b, 0x01};
+    fwrite(magic, 1, 3, f);
+    const int64_t np = (g->M + 3) / 4;
+    const int64_t chunk = std::max<int64_t>(1, (64ll << 20) / np);
+    std::vector<uint8_t> buf((size_t)(std::min(chunk, (int64_t)g->N) * np));
+    for (int64_t s0 = 0; s0 < g->N; s0 += chunk) {
+        const int64_t n = std::min(chunk, (int64_t)g->N - s0);
+        gen_rows_host(spec, tab, s0, n, buf.data(), host_threads());
+        fwrite(buf.data(), 1, (size_t)(n * np), f);
+    }
+    fclose(f);
+    gen_free(&tab);
+    return 0;
+}
+
+int dpca_get_dims(dpca_ctx *c, int *M, long long *N, int *nsv, int *nrhs, int *bsize)
+{
+    if (!c) return -1;
+    if (M) *M = c->M;
+    if (N) *N = c->N;
+    if (nsv) *nsv = c->nsv;
+    if (nrhs) *nrhs = c->k;
+    if (bsize) *bsize = c->bsize;
+    return 0;
+}
+
+int dpca_snp_stats(dpca_ctx *c, long long *sum, long long *ctr, double *zval)
+{
+    if (!c) return -1;
+    (void)hipSetDevice(c->dev);
+    if (sum) DPCA_CHECK(hipMemcpy(sum, c->ssum, (size_t)c->N * 8, hipMemcpyDeviceToHost));
+    if (ctr) DPCA_CHECK(hipMemcpy(ctr, c->sctr, (size_t)c->N * 8, hipMemcpyDeviceToHost));
+    if (zval) DPCA_CHECK(hipMemcpy(zval, c->zval, (size_t)c->N * 32, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dpca_decode(dpca_ctx *c, long long snp0, long long rows, uint8_t *out)
+{
+    if (!c || !out || snp0 < 0 || rows < 0 || snp0 + rows > c->N) {
+        set_error("dpca_decode: bad arguments");
+        return -1;
+    }
+    (void)hipSetDevice(c->dev);
+    uint8_t *d = nullptr;
+    if (dalloc(&d, (size_t)rows * c->M)) return -3;
+    hipError_t e = launch_decode_codes(c->st, c->packed + snp0 * c->ldp, rows, c->ldp, c->M, d);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d, (size_t)rows * c->M, hipMemcpyDeviceToHost, c->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        set_error("dpca_decode: %s", hipGetErrorString(e));
+        return -2;
+    }
+    return 0;
+}
+
+int dpca_copy_packed(dpca_ctx *c, long long snp0, long long rows, uint8_t *out)
+{
+    if (!c || !out || snp0 < 0 || rows < 0 || snp0 + rows > c->N) {
+        set_error("dpca_copy_packed: bad arguments");
+        return -1;
+    }
+    (void)hipSetDevice(c->dev);
+    DPCA_CHECK(hipMemcpy2D(out, c->np, c->packed + snp0 * c->ldp, c->ldp, c->np, rows,
+                           hipMemcpyDeviceToHost));
+    // the device copy holds missing codes past sample M; the file holds 00
+    const int valid = c->M - 4 * (int)(c->np - 1);
+    if (valid < 4) {
+        const uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
+        for (long long r = 0; r < rows; r++) out[r * c->np + c->np - 1] &= keep;
+    }
+    return 0;
+}
+
+int dpca_pass(dpca_ctx *c, const double *Q, double *AAtQ)
+{
+    if (!c || !Q || !AAtQ) {
+        set_error("dpca_pass: NULL argument");
+        return -1;
+    }
+    (void)hipSetDevice(c->dev);
+    int r;
+    if ((r = copy_in_rhs(c, Q, c->RHS2))) return r;
+    if ((r = do_pass(c, c->RHS2, c->ARHS))) return r;
+    if ((r = copy_out_rhs(c, c->ARHS, AAtQ))) return r;
+    collect_timers(c);
+    return 0;
+}
+
+int dpca_run(dpca_ctx *c, double *rhs_inout, double *sigma, dpca_report *report)
+{
+    if (!c) {
+        set_error("dpca_run: NULL context");
+        return -1;
+    }
+    (void)hipSetDevice(c->dev);
+    return run(c, rhs_inout, sigma, report);
+}
+
+int dpca_crit_history(dpca_ctx *c, double *vals, int cap)
+{
+    if (!c) return -1;
+    int n = std::min<int>(cap, (int)c->crit_hist.size());
+    for (int i = 0; i < n; i++) vals[i] = c->crit_hist[i];
+    return n;
+}
+
+int dpca_write_outputs(dpca_ctx *c, const char *prefix, const double *rhs, const double *sigma)
+{
+    if (!c || !prefix || !rhs || !sigma) {
+        set_error("dpca_write_outputs: NULL argument");
+        return -1;
+    }
+    const std::string base(prefix);
+    FILE *f = fopen((base + "_singularValues.txt").c_str(), "w");
+    if (!f) {
+        set_error("fopen singularValues: %s", strerror(errno));
+        return -1;
+    }
+    for (int j = 0; j < c->nsv; j++) fprintf(f, "%2.13lf\n", sigma[j]);
+    fclose(f);
+    f = fopen((base + "_leftSingularVectors.txt").c_str(), "w");
+    if (!f) {
+        set_error("fopen leftSingularVectors: %s", strerror(errno));
+        return -1;
+    }
+    fprintf(f, "FID");
+    for (int j = 0; j < c->nsv; j++) fprintf(f, "\tPC%d", j);
+    fprintf(f, "\n");
+    for (int i = 0; i < c->M; i++) {
+        fprintf(f, "%8s", c->iids[i].c_str());
+        for (int j = 0; j < c->nsv; j++) fprintf(f, "\t%2.13f", rhs[(size_t)i * c->k + j]);
+        fprintf(f, "\n");
+    }
+    fclose(f);
+    return 0;
+}
+
+int dpca_orthonormalize(const double *A, long long m, int k, double *Q, int *shifted, int device)
+{
+    if (!A || !Q || m <= 0 || k <= 0 || k > 64 || k > m) {
+        set_error("dpca_orthonormalize: bad arguments");
+        return -1;
+    }
+    DPCA_CHECK(hipSetDevice(device));
+    dpca_ctx c;
+    c.M = (int)std::min<long long>(m, 0x7fffffff);
+    c.k = k;
+    c.kp = (int)round_up(k, 16);
+    c.Mpad = m;
+    DPCA_CHECK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+    const size_t bytes = (size_t)m * c.kp * 8;
+    int r = dalloc(&c.tmp, bytes) | dalloc(&c.Q, bytes) | dalloc(&c.ARHS, bytes) |
+            dalloc(&c.gpart, (size_t)c.ngparts * 64 * 64 * 8) | dalloc(&c.G, 64 * 64 * 8) |
+            dalloc(&c.Rinv, 64 * 64 * 8) | dalloc(&c.flags, 64);
+    int fl[16] = {0};
+    if (!r) {
+        hipError_t e = hipMemsetAsync(c.ARHS, 0, bytes, c.st);
+        if (e == hipSuccess) e = hipMemsetAsync(c.flags, 0, 64, c.st);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(c.ARHS, c.kp * 8, A, k * 8, k * 8, m, hipMemcpyHostToDevice, c.st);
+        if (e != hipSuccess) r = -2;
+    }
+    if (!r) r = orthonormalise(&c, c.ARHS);
+    if (!r) {
+        hipError_t e = hipMemcpy2DAsync(Q, k * 8, c.Q, c.kp * 8, k * 8, m, hipMemcpyDeviceToHost, c.st);
+        if (e == hipSuccess) e = hipMemcpyAsync(fl, c.flags, sizeof fl, hipMemcpyDeviceToHost, c.st);
+        if (e == hipSuccess) e = hipStreamSynchronize(c.st);
+        if (e != hipSuccess) {
+            set_error("dpca_orthonormalize: %s", hipGetErrorString(e));
+            r = -2;
+        }
+    }
+    collect_timers(&c);
+    for (void *p : {(void *)c.tmp, (void *)c.Q, (void *)c.ARHS, (void *)c.gpart, (void *)c.G,
+                    (void *)c.Rinv, (void *)c.flags})
+        if (p) (void)hipFree(p);
+    c.tmp = c.Q = c.ARHS = c.gpart = c.G = c.Rinv = nullptr;
+    c.flags = nullptr;
+    for (auto e : c.free_ev) hipEventDestroy(e);
+    c.free_ev.clear();
+    (void)hipStreamDestroy(c.st);
+    c.st = nullptr;
+    if (shifted) *shifted = fl[0];
+    if (!r && fl[1]) {
+        set_error("CholeskyQR failed even with the shifted fallback");
+        r = -4;
+    }
+    return r;
+}
+
+int dpca_syev(const double *B, int k, double *w, double *X, int device)
+{
+    if (!B || !w || !X || k <= 0 || k > 64) {
+        set_error("dpca_syev: bad arguments");
+        return -1;
+    }
+    DPCA_CHECK(hipSetDevice(device));
+    const int kp = (int)round_up(k, 16);
+    double *dB = nullptr, *dw = nullptr, *dX = nullptr;
+    int *ds = nullptr;
+    if (dalloc(&dB, kp * kp * 8) | dalloc(&dw, 64 * 8) | dalloc(&dX, kp * kp * 8) | dalloc(&ds, 64))
+        return -3;
+    std::vector<double> hb((size_t)kp * kp, 0.0), hx((size_t)kp * kp);
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) hb[(size_t)i * kp + j] = B[(size_t)i * k + j];
+    int st = 0;
+    hipError_t e = hipMemcpy(dB, hb.data(), hb.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(ds, 0, 64);
+    if (e == hipSuccess) e = launch_syev(nullptr, dB, k, kp, dw, dX, ds);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(w, dw, k * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(hx.data(), dX, hx.size() * 8, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&st, ds, sizeof st, hipMemcpyDeviceToHost);
+    (void)hipFree(dB);
+    (void)hipFree(dw);
+    (void)hipFree(dX);
+    (void)hipFree(ds);
+    if (e != hipSuccess) {
+        set_error("dpca_syev: %s", hipGetErrorString(e));
+        return -2;
+    }
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) X[(size_t)i * k + j] = hx[(size_t)i * kp + j];
+    if (st) {
+        set_error("Jacobi eigensolver did not converge");
+        return -4;
+    }
+    return 0;
+}
+
+void dpca_close(dpca_ctx *c) { free_ctx(c); }
+
+}  // extern "C"

@@ -77,7 +77,7 @@ hipError_t launch_mask_fix_a(hipStream_t st, const float *HQ, const double *mu,
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
                        double *part, int nparts, double *G);
 hipError_t launch_chol_inv(hipStream_t st, const double *G, int k, int kp, double shift_mult,
-                           int64_t m, double *Rinv, int *status);
+                           int mode, double *Rinv, int *flags);
 hipError_t launch_apply(hipStream_t st, const double *X, const double *W, int64_t m, int kp,
                         double *Y);
 hipError_t launch_syev(hipStream_t st, const double *B, int k, int kp, double *w, double *V,
@@ -86,6 +86,16 @@ hipError_t launch_col_dots(hipStream_t st, const double *X, const double *Y, int
                            int kp, int ncols, double *part, int nparts, double *dots);
 hipError_t launch_colsum(hipStream_t st, const double *X, int64_t m, int kp, double *part,
                          int nparts, double *out);
+
+// Per-iteration state of the convergence test (device resident).
+struct IterState {
+    double sigma[64];
+    double sigma_old[64];
+    double delta, delta_prev, crit;
+    int conv, stop;
+};
+hipError_t launch_crit(hipStream_t st, const double *w, int nsv, int crit, double tol, int ii,
+                       const double *dots, IterState *s);
 
 // Synthetic generator (device side).  See dpca_gen.h.
 struct GenSpec;

@@ -83,39 +83,22 @@ hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t
 }
 
 // -------------------------------------------------- Cholesky + R^{-1} ------
-// G (kp x kp, leading k x k used) = R^T R with R upper triangular; writes
-// R^{-1} (kp x kp, zero outside the leading block).  With shift_mult > 0 the
-// diagonal is first raised by shift_mult * trace(G) (shifted CholeskyQR,
-// Fukaya et al., SIAM J. Sci. Comput. 42 (2020)).  status = 1 when a pivot
-// is not positive.
-__global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, int k,
-                                                  int kp, double shift_mult,
-                                                  double *__restrict__ Rinv, int *status)
+// G (kp x kp, leading k x k used; its upper triangle defines the symmetric
+// matrix) = R^T R with R upper triangular; writes R^{-1} (kp x kp, zero
+// outside the leading block).  CholQR2 with the shifted-CholeskyQR3 fallback
+// of Fukaya et al., SIAM J. Sci. Comput. 42 (2020):
+//   mode 0  factor G; if a pivot is not positive, raise the diagonal by
+//           shift_mult * trace(G), factor again and set flags[0] = 1;
+//   mode 1  factor G;
+//   mode 2  third pass: R^{-1} = I unless flags[0] says the shift was used.
+// A failure that survives the shift sets flags[1] = 1.
+__device__ int chol_factor(double (*A)[kMaxKp + 1], int k, int *bad)
 {
-    __shared__ double A[kMaxKp][kMaxKp + 1];
-    __shared__ double col[kMaxKp];
-    __shared__ int bad;
     const int tid = threadIdx.x;
-    for (int e = tid; e < kp * kp; e += 256) {
-        const int i = e / kp, j = e % kp;
-        // the upper triangle defines the symmetric matrix
-        A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
-    }
-    if (tid == 0) bad = 0;
-    __syncthreads();
-    if (shift_mult > 0) {
-        if (tid == 0) {
-            double tr = 0;
-            for (int i = 0; i < k; i++) tr += A[i][i];
-            const double sh = shift_mult * tr;
-            for (int i = 0; i < k; i++) A[i][i] += sh;
-        }
-        __syncthreads();
-    }
     for (int j = 0; j < k; j++) {
         if (tid == 0) {
             const double d = A[j][j];
-            if (!(d > 0.0)) bad = 1;
+            if (!(d > 0.0)) *bad = 1;
             A[j][j] = sqrt(d > 0.0 ? d : 1.0);
         }
         __syncthreads();
@@ -128,6 +111,51 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
             if (r <= c) A[r][c] -= A[j][r] * A[j][c];
         }
         __syncthreads();
+    }
+    return *bad;
+}
+
+__global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, int k,
+                                                  int kp, double shift_mult, int mode,
+                                                  double *__restrict__ Rinv, int *flags)
+{
+    __shared__ double A[kMaxKp][kMaxKp + 1];
+    __shared__ double col[kMaxKp];
+    __shared__ int bad, skip;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        bad = 0;
+        skip = (mode == 2 && flags[0] == 0);
+    }
+    __syncthreads();
+    if (skip) {
+        for (int e = tid; e < kp * kp; e += 256)
+            Rinv[e] = (e / kp == e % kp && e / kp < k) ? 1.0 : 0.0;
+        return;
+    }
+    for (int e = tid; e < kp * kp; e += 256) {
+        const int i = e / kp, j = e % kp;
+        A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
+    }
+    __syncthreads();
+    chol_factor(A, k, &bad);
+    if (bad && mode == 0) {
+        __syncthreads();
+        for (int e = tid; e < kp * kp; e += 256) {
+            const int i = e / kp, j = e % kp;
+            A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double tr = 0;
+            for (int i = 0; i < k; i++) tr += A[i][i];
+            const double sh = shift_mult * tr;
+            for (int i = 0; i < k; i++) A[i][i] += sh;
+            bad = 0;
+            flags[0] = 1;
+        }
+        __syncthreads();
+        chol_factor(A, k, &bad);
     }
     // in-place inverse of the upper triangle, one column at a time
     for (int j = 0; j < k; j++) {
@@ -147,14 +175,14 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
         const int i = e / kp, j = e % kp;
         Rinv[e] = (i < k && j < k && i <= j) ? A[i][j] : 0.0;
     }
-    if (tid == 0) *status = bad;
+    if (tid == 0 && bad) flags[1] = 1;
 }
 
 hipError_t launch_chol_inv(hipStream_t st, const double *G, int k, int kp, double shift_mult,
-                           int64_t, double *Rinv, int *status)
+                           int mode, double *Rinv, int *flags)
 {
     if (kp > kMaxKp || k > kp) return hipErrorInvalidValue;
-    k_chol_inv<<<1, 256, 0, st>>>(G, k, kp, shift_mult, Rinv, status);
+    k_chol_inv<<<1, 256, 0, st>>>(G, k, kp, shift_mult, mode, Rinv, flags);
     return hipGetLastError();
 }
 
@@ -242,7 +270,7 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ 
             if (tid < o) red[tid] += red[tid + o];
             __syncthreads();
         }
-        if (tid == 0) done = !(offs > 1e-34 * red[0]) || offs == 0.0;
+        if (tid == 0) done = !(offs > 1e-30 * red[0]);
         __syncthreads();
         if (done) break;
         for (int round = 0; round < n - 1; round++) {
@@ -318,7 +346,7 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ 
         if (j >= k) V[e] = 0.0;
         (void)i;
     }
-    if (tid == 0) *status = (sweeps >= 40) ? 1 : 0;
+    if (tid == 0 && sweeps >= 40) *status = 1;
 }
 
 hipError_t launch_syev(hipStream_t st, const double *B, int k, int kp, double *w, double *V,
@@ -369,6 +397,62 @@ hipError_t launch_colsum(hipStream_t st, const double *X, int64_t m, int kp, dou
                          int nparts, double *out)
 {
     return launch_col_dots(st, X, nullptr, m, kp, kp, part, nparts, out);
+}
+
+}  // namespace dpca
+
+namespace dpca {
+
+// ------------------------------------------------- convergence criteria ----
+// sigma_j = sqrt(w_j) for the nsv largest Ritz values and the three stopping
+// tests of src/methods.cpp:1012-1056 (crit 0: relative change of the sum of
+// singular values; crit 1: every |(s_j - s_j_old)/s_j| <= tol; crit 2:
+// 1 - (1/nsv) sum_j (q_old_j . q_new_j)^2 <= tol), evaluated on the device in
+// the reference's operation order.
+__global__ void k_crit(const double *__restrict__ w, int nsv, int crit, double tol, int ii,
+                       const double *__restrict__ dots, IterState *s)
+{
+#pragma clang fp contract(off)
+    if (threadIdx.x != 0) return;
+    double delta = 0.0;
+    for (int j = 0; j < nsv; j++) {
+        s->sigma[j] = sqrt(w[j]);
+        delta += s->sigma[j];
+    }
+    int stop = 0, conv = 0;
+    double cval = 0.0;
+    if (ii > 0) {
+        if (crit == 0) {
+            cval = fabs(s->delta_prev - delta) / delta;
+            stop = cval <= tol;
+        } else if (crit == 1) {
+            for (int j = 0; j < nsv; j++)
+                if (fabs((s->sigma[j] - s->sigma_old[j]) / s->sigma[j]) <= tol) conv++;
+            cval = conv;
+            stop = conv == nsv;
+        } else if (crit == 2) {
+            double mev = 0.0;
+            for (int j = 0; j < nsv; j++) mev += dots[j] * dots[j];
+            mev /= nsv;
+            cval = 1.0 - mev;
+            stop = cval <= tol;
+        }
+    }
+    s->delta = delta;
+    s->crit = cval;
+    s->conv = conv;
+    s->stop = stop;
+    s->delta_prev = delta;
+    if (!stop)
+        for (int j = 0; j < nsv; j++) s->sigma_old[j] = s->sigma[j];
+}
+
+hipError_t launch_crit(hipStream_t st, const double *w, int nsv, int crit, double tol, int ii,
+                       const double *dots, IterState *s)
+{
+    if (nsv > kMaxKp) return hipErrorInvalidValue;
+    k_crit<<<1, 64, 0, st>>>(w, nsv, crit, tol, ii, dots, s);
+    return hipGetLastError();
 }
 
 }  // namespace dpca

@@ -1,0 +1,157 @@
+/*
+ * libdpca — C ABI of the MI355X-native randomized block subspace-iteration
+ * PCA over PLINK .bed genotype matrices.
+ *
+ * The reference (CEID-HPCLAB/DistPCA) has no library interface: its boundary
+ * is the DistPCA.exe process (src/distpca.cpp:47-644) and, inside it, the C++
+ * call BlockSubspaceIter_MPI_OOC_double_buffering(const char*, double*,
+ * logistics*) (src/methods.h:49, src/methods.cpp:702-1135).  Each entry point
+ * below names the reference code it replaces.  Plain pointers and sizes only;
+ * every call returns 0 on success or a negative code, and dpca_last_error()
+ * holds the message (the CLI prints it and exits 1, as the reference does).
+ * Nothing in the library aborts the process.
+ *
+ * Matrices are row-major, M x k (samples x right-hand sides), exactly as the
+ * reference's RHS2/ARHS buffers.  One context per host thread.
+ */
+#ifndef DPCA_H
+#define DPCA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dpca_ctx dpca_ctx;
+
+enum { DPCA_FP64 = 0, DPCA_MIXED = 1 };
+
+/* Run parameters.  Field meaning and defaults follow struct logistics and
+ * initialize_structure (src/structures.h:54-102, src/utilities.cpp:195-232):
+ * nsv 10, nrhs 20, power 1, crit 1, tol 1e-3, miter 100, bsize 0 (auto),
+ * verbose 1, fwrite 0, fullsvd 0, memory 0, prefix "noprefix". */
+typedef struct dpca_params {
+    int nsv, nrhs, power, crit;
+    double tol;
+    int bsize, miter, verbose, fwrite, fullsvd;
+    double memory;          /* -memory, GB (0 = RAM/10 as the reference)    */
+    char prefix[1024];      /* -prefix                                       */
+    int precision;          /* DPCA_FP64 (plain form) or DPCA_MIXED          */
+    int device;             /* HIP device ordinal                            */
+    long long device_cap;   /* bytes of HBM the packed matrix may use (0 =
+                               no cap); above it the matrix streams from
+                               pinned host memory in bsize-SNP blocks        */
+    int strict_passes;      /* 1 (default): power+1 data passes per outer
+                               iteration exactly as the reference            */
+} dpca_params;
+
+/* Timers and counters of one run.  The t_* fields are the reference's
+ * TIME_2_* stage timers (src/structures.h:87-88), in seconds; `wall` is the
+ * wall clock of dpca_run itself. */
+typedef struct dpca_report {
+    int M;
+    long long N;
+    int nsv, nrhs, power, bsize, crit;
+    int total_its;          /* blockPower_total_its                          */
+    int passes;             /* data passes (fetches) performed               */
+    int incore;             /* 1 when bsize >= N (SubspaceIteration_MPI)     */
+    int streamed;           /* 1 when the matrix streamed from host memory   */
+    int qr_shifted;         /* CholQR needed its shifted fallback            */
+    double t_generate_rhs, t_load, t_mm, t_mm_a, t_mm_at, t_gs, t_rr, t_other;
+    double wall;
+    double last_crit;       /* last criterion value (crit 0/2) or count (1)  */
+} dpca_report;
+
+/* Defaults of initialize_structure (src/utilities.cpp:195-232). */
+int dpca_params_init(dpca_params *p);
+
+/* Replaces the file handling of main (src/distpca.cpp:96-189, 239-352):
+ * reads <bfile>.fam / .bim (newline-terminated lines count, as
+ * src/distpca.cpp:137-147), opens <bfile>.bed, applies the bsize and
+ * NSV/NRHS clamps (src/distpca.cpp:239-352) and stages the packed matrix in
+ * HBM (or in pinned host memory when it exceeds device_cap). */
+int dpca_open(const char *bfile, const dpca_params *p, dpca_ctx **out);
+
+/* Same as dpca_open for a packed matrix already in host memory in .bed row
+ * layout (N rows of ceil(M/4) bytes, no magic).  IDs are "<i>" for samples. */
+int dpca_open_packed(const uint8_t *packed, int M, long long N, const dpca_params *p,
+                     dpca_ctx **out);
+
+/* Synthetic Balding-Nichols genotypes generated directly in HBM
+ * (no reference counterpart; used for the documented workloads). */
+typedef struct dpca_gen_spec {
+    int M;
+    long long N;
+    int npop;
+    double fst, admix_frac, missing;
+    unsigned long long seed;
+    double pmin, pmax;      /* ancestral allele frequency range              */
+} dpca_gen_spec;
+int dpca_open_synthetic(const dpca_gen_spec *g, const dpca_params *p, dpca_ctx **out);
+/* Writes <prefix>.bed/.bim/.fam with the same generator on the host. */
+int dpca_write_synthetic(const char *prefix, const dpca_gen_spec *g);
+
+/* Resolved sizes after the clamps. */
+int dpca_get_dims(dpca_ctx *c, int *M, long long *N, int *nsv, int *nrhs, int *bsize);
+
+/* Reference-exact Omega, M x k row-major: the loop of src/distpca.cpp:384-397
+ * over rand_val / norm2 (src/gennorm.c:21-72), including the int32 seed
+ * wrap for M >= 46341.  Host-side (its log/cos are the host libm's). */
+int dpca_omega(int M, int k, double *out);
+
+/* Per-SNP statistics of the resident matrix, i.e. what standardize and the
+ * norm_precomp cache compute (src/utilities.cpp:272-306,
+ * src/methods.cpp:819-843): integer sum and non-missing count, and
+ * zval[4*s + code] = ((g - avg)/sd) * (1/sqrt(N)) for the 2-bit code
+ * (00 -> g=0, 10 -> g=1, 11 -> g=2, 01 missing -> 0).  Any pointer may be
+ * NULL. */
+int dpca_snp_stats(dpca_ctx *c, long long *sum, long long *ctr, double *zval);
+
+/* Decoded codes of rows [snp0, snp0+rows) through the {0,3,1,2} table of
+ * decode_plink_sse2 (src/utilities.cpp:351-414): out is rows x M bytes. */
+int dpca_decode(dpca_ctx *c, long long snp0, long long rows, uint8_t *out);
+
+/* Packed rows back in .bed layout (rows x ceil(M/4) bytes). */
+int dpca_copy_packed(dpca_ctx *c, long long snp0, long long rows, uint8_t *out);
+
+/* One data pass AAtQ = A A^T Q = Z^T (Z Q): the lambda _compute_matvec_buff
+ * (src/methods.cpp:786-936).  Q, AAtQ: M x nrhs host buffers. */
+int dpca_pass(dpca_ctx *c, const double *Q, double *AAtQ);
+
+/* The subspace iteration, BlockSubspaceIter_MPI_OOC_double_buffering
+ * (src/methods.cpp:938-1075) or, when bsize >= N, SubspaceIteration_MPI
+ * (src/methods.cpp:24-267).  rhs_inout: M x nrhs, Omega on entry (NULL:
+ * generated with dpca_omega), the Ritz vectors on return.  sigma: nsv
+ * singular values (sqrt of the Ritz values), descending.  report may be
+ * NULL. */
+int dpca_run(dpca_ctx *c, double *rhs_inout, double *sigma, dpca_report *report);
+
+/* Per-iteration criterion values of the last run (crit 0/2: value, crit 1:
+ * converged count), as printed at -verbose 2 (src/methods.cpp:1025-1053).
+ * Returns the number written (<= cap). */
+int dpca_crit_history(dpca_ctx *c, double *vals, int cap);
+
+/* <prefix>_singularValues.txt and <prefix>_leftSingularVectors.txt in the
+ * formats of src/methods.cpp:1077-1118. */
+int dpca_write_outputs(dpca_ctx *c, const char *prefix, const double *rhs,
+                       const double *sigma);
+
+/* Orthonormalisation unit (replaces LAPACKE_dgeqrf + dorgqr,
+ * src/methods.cpp:947-948): CholQR2 on the device, Q = A R^{-1}.  A, Q:
+ * m x k host buffers; returns 1 in *shifted when the shifted fallback ran. */
+int dpca_orthonormalize(const double *A, long long m, int k, double *Q, int *shifted,
+                        int device);
+
+/* Symmetric eigensolver unit (replaces LAPACKE_dsyev('V','U') plus the column
+ * reversal, src/methods.cpp:980-1000): upper triangle of the k x k B;
+ * eigenvalues descending in w, unit eigenvectors as the columns of X. */
+int dpca_syev(const double *B, int k, double *w, double *X, int device);
+
+void dpca_close(dpca_ctx *c);
+const char *dpca_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DPCA_H */

@@ -1,0 +1,199 @@
+"""GPU parity of the plain FP64 form against the oracle, through the C ABI.
+
+Tolerances (FP64 form vs the reference's CPU arithmetic, restated by the
+oracle):  decode and per-SNP statistics bit-exact; one pass within 1e-12 of
+the pass norm; eigenvalues (sigma^2) within 1e-10 relative; principal angles
+of the top-nsv subspace within 1e-8 rad where the iteration has converged.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import distpca_amd as D
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def _packed_random(M, N, seed, miss=0.02):
+    rng = np.random.default_rng(seed)
+    codes = rng.choice([0, 1, 2, 3], size=(N, M), p=[0.4, miss, 0.35, 0.25 - miss]).astype(np.uint8)
+    codes[min(3, N - 1)] = 0   # monomorphic
+    codes[min(5, N - 1)] = 1   # all missing
+    np_ = (M + 3) // 4
+    pad = np.zeros((N, 4 * np_), np.uint8)
+    pad[:, :M] = codes
+    packed = pad[:, 0::4] | (pad[:, 1::4] << 2) | (pad[:, 2::4] << 4) | (pad[:, 3::4] << 6)
+    return np.ascontiguousarray(packed.astype(np.uint8))
+
+
+def test_stats_and_decode_bitexact_hgdp(hgdp_prefix, hgdp_golden):
+    _banner("stats/decode HGDP 942 x 133594")
+    with D.PCA.from_bfile(hgdp_prefix, D.Params(nsv=20, nrhs=40, bsize=100)) as pca:
+        s, c, z = pca.snp_stats()
+        assert np.array_equal(s, hgdp_golden["sum"])
+        assert np.array_equal(c, hgdp_golden["ctr"])
+        assert np.array_equal(z.view(np.int64), hgdp_golden["zval"].view(np.int64))
+        M, N, _, _, packed = O.read_plink(hgdp_prefix)
+        rows = pca.decode(1000, 64)
+        np_ = (M + 3) // 4
+        for r in range(64):
+            ref = O.decode(packed[(1000 + r) * np_:(1001 + r) * np_])[:M]
+            assert np.array_equal(rows[r], ref)
+        back = pca.packed(0, 300)
+        assert np.array_equal(back.reshape(-1), packed[:300 * np_])
+
+
+@pytest.mark.parametrize("M,N", [(1001, 777), (5, 300), (256, 256), (1024, 50000)])
+def test_stats_edge_shapes(M, N):
+    _banner(f"stats edge M={M} N={N}")
+    packed = _packed_random(M, N, seed=M + N)
+    s0, c0, a0, d0, z0 = O.snp_stats(packed, N, M)
+    with D.PCA.from_packed(packed, M, N, D.Params(nsv=2, nrhs=4, bsize=64)) as pca:
+        s, c, z = pca.snp_stats()
+    assert np.array_equal(s, s0) and np.array_equal(c, c0)
+    assert np.array_equal(z.view(np.int64), z0.view(np.int64))
+
+
+def test_stats_every_count_pair():
+    # zval depends only on (sum, ctr): cover every pair reachable at M = 64
+    M = 64
+    rows = []
+    for ctr in range(0, M + 1):
+        for sm in range(0, 2 * ctr + 1):
+            n2 = max(0, sm - ctr)
+            n1 = sm - 2 * n2
+            n0 = ctr - n1 - n2
+            codes = [0] * n0 + [2] * n1 + [3] * n2 + [1] * (M - ctr)
+            rows.append(codes)
+    codes = np.array(rows, np.uint8)
+    N = codes.shape[0]
+    packed = (codes[:, 0::4] | (codes[:, 1::4] << 2) | (codes[:, 2::4] << 4) |
+              (codes[:, 3::4] << 6)).astype(np.uint8)
+    s0, c0, a0, d0, z0 = O.snp_stats(packed, N, M)
+    with D.PCA.from_packed(packed, M, N, D.Params(nsv=2, nrhs=4, bsize=64)) as pca:
+        s, c, z = pca.snp_stats()
+    assert np.array_equal(z.view(np.int64), z0.view(np.int64))
+
+
+@pytest.mark.parametrize("M,N,k", [(1001, 3000, 20), (942, 5000, 40), (300, 1000, 7)])
+def test_pass_matches_oracle(M, N, k):
+    _banner(f"pass M={M} N={N} k={k}")
+    packed = _packed_random(M, N, seed=k)
+    Q = np.random.default_rng(k).standard_normal((M, k))
+    ref = O.OOCPass(packed, M, N, 512)(Q)
+    with D.PCA.from_packed(packed, M, N, D.Params(nsv=min(k, 5), nrhs=k, bsize=512)) as pca:
+        got = pca.pass_(Q)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-12, err
+
+
+def test_orthonormalize_cholqr2():
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((5000, 40)) @ np.diag(np.logspace(0, 4, 40))
+    Q, shifted = D.orthonormalize(A)
+    assert not shifted
+    assert np.abs(Q.T @ Q - np.eye(40)).max() < 1e-13
+    R = Q.T @ A
+    assert np.abs(np.tril(R, -1)).max() < 1e-9 * np.abs(R).max()
+    assert np.all(np.diag(R) > 0)
+    # ill-conditioned input: the shifted fallback keeps Q orthonormal
+    B = rng.standard_normal((4000, 30)) @ np.diag(np.logspace(0, 12, 30))
+    Q2, shifted2 = D.orthonormalize(B)
+    assert shifted2
+    assert np.abs(Q2.T @ Q2 - np.eye(30)).max() < 1e-12
+    qn, _ = np.linalg.qr(B)
+    assert O.principal_angles(qn, Q2).max() < 1e-6
+
+
+@pytest.mark.parametrize("k", [1, 7, 20, 40, 64])
+def test_syev_jacobi(k):
+    rng = np.random.default_rng(k)
+    S = rng.standard_normal((k, k))
+    S = S @ S.T + np.diag(np.arange(k, dtype=float))
+    B = np.triu(S) + np.tril(rng.standard_normal((k, k)), -1)  # lower part must be ignored
+    w, X = D.syev(B)
+    ref = np.linalg.eigvalsh(S)[::-1]
+    assert np.allclose(w, ref, rtol=1e-12, atol=1e-12 * ref[0])
+    assert np.abs(X.T @ X - np.eye(k)).max() < 1e-12
+    assert np.abs(S @ X - X * w).max() < 1e-11 * ref[0]
+
+
+def test_run_hgdp_matches_reference_settings(hgdp_prefix, hgdp_golden):
+    # example/example.sh:5-12 settings; the published log implies 11 iterations
+    _banner("run HGDP nsv 20 nrhs 40 crit 2 tol 1e-3 bsize 100")
+    p = D.Params(nsv=20, nrhs=40, crit=2, tol=1e-3, bsize=100)
+    with D.PCA.from_bfile(hgdp_prefix, p) as pca:
+        sigma, rhs, rep = pca.run()
+        crit = pca.crit_history()
+    assert rep["total_its"] == 11 and rep["passes"] == 22
+    assert np.allclose(crit, hgdp_golden["crit"], rtol=1e-6, atol=1e-12)
+    rel = np.abs(sigma ** 2 - hgdp_golden["sigma"] ** 2) / hgdp_golden["sigma"] ** 2
+    print("HGDP eigenvalue rel err max", rel.max())
+    assert rel.max() < 1e-10
+    ang = O.principal_angles(rhs[:, :20], hgdp_golden["rhs"][:, :20])
+    print("HGDP principal angles max", ang.max())
+    assert ang.max() < 1e-8
+
+
+def test_run_c1_synthetic(c1_golden):
+    _banner("run c1 1024 x 50000 on device-generated data")
+    spec = D.gen_spec(1024, 50000, npop=3, fst=0.05, missing=0.01, seed=0)
+    p = D.Params(nsv=10, nrhs=20, power=4, crit=1, tol=1e-6, miter=20, bsize=4096)
+    with D.PCA.synthetic(spec, p) as pca:
+        import hashlib
+        sha = np.frombuffer(hashlib.sha256(pca.packed().tobytes()).digest(), np.uint8)
+        assert np.array_equal(sha, c1_golden["bed_sha"]), "device generator != host generator"
+        sigma, rhs, rep = pca.run()
+    assert rep["total_its"] == int(c1_golden["total_its"])
+    rel = np.abs(sigma ** 2 - c1_golden["sigma"] ** 2) / c1_golden["sigma"] ** 2
+    ang = O.principal_angles(rhs[:, :10], c1_golden["rhs"][:, :10])
+    print("c1 eigenvalue rel err", rel.max(), "angles", ang)
+    assert rel.max() < 1e-10
+    # PCs 1-2 carry the 3-population structure and converge; PCs 3-10 sit in
+    # the noise bulk and are not converged after 20 iterations
+    assert O.principal_angles(rhs[:, :2], c1_golden["rhs"][:, :2]).max() < 1e-8
+
+
+def test_incore_variant_matches_oracle():
+    M, N = 400, 2000
+    packed = _packed_random(M, N, seed=11)
+    p = D.Params(nsv=5, nrhs=10, power=2, crit=1, tol=1e-12, miter=60, bsize=N)
+    Z = O.dense_z(packed, N, M)
+    ref = O.rsi_incore(Z, O.resolve(O.Params(nsv=5, nrhs=10, power=2, crit=1, tol=1e-12,
+                                             miter=60, bsize=N), M, N))
+    with D.PCA.from_packed(packed, M, N, p) as pca:
+        sigma, rhs, rep = pca.run()
+    assert rep["incore"] == 1 and rep["total_its"] == ref.total_its
+    assert np.allclose(sigma, ref.sigma, rtol=1e-12)
+
+
+def test_cli_dropin(tmp_path):
+    pre = str(tmp_path / "cli")
+    spec = D.gen_spec(600, 4000, npop=3, fst=0.05, missing=0.01, seed=5)
+    D.write_synthetic(pre, spec)
+    out = subprocess.run([D.EXE_PATH, "-bfile", pre, "-nsv", "4", "-nrhs", "8", "-power", "2",
+                          "-crit", "2", "-tol", "1e-6", "-bsize", "500", "-fwrite", "1",
+                          "-prefix", str(tmp_path / "res"), "-verbose", "2"],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = out.stdout.splitlines()
+    i = lines.index("[The following times are listed in seconds]")
+    tline = next(l for l in lines[i:] if l.startswith("Total wall-clock time elapsed:"))
+    float(tline.split()[4])  # field 5, as example/example.sh parses it
+    sv = np.loadtxt(str(tmp_path / "res_singularValues.txt"))
+    M, N, iids, _, packed = O.read_plink(pre)
+    r = O.rsi(packed, M, N, O.resolve(O.Params(nsv=4, nrhs=8, power=2, crit=2, tol=1e-6,
+                                               bsize=500), M, N))
+    assert np.allclose(sv, r.sigma, rtol=1e-10)
+    txt = open(str(tmp_path / "res_leftSingularVectors.txt")).read().splitlines()
+    assert txt[0] == "FID\tPC0\tPC1\tPC2\tPC3"
+    assert txt[1].split("\t")[0].strip() == iids[0] and len(txt) == M + 1
+    U = np.array([[float(x) for x in l.split("\t")[1:]] for l in txt[1:]])
+    assert O.principal_angles(U, r.rhs[:, :4]).max() < 1e-8
