@@ -63,7 +63,7 @@ EXPORTS = [
     "dpca_params_init", "dpca_open", "dpca_open_packed", "dpca_open_synthetic",
     "dpca_write_synthetic", "dpca_get_dims", "dpca_omega", "dpca_snp_stats", "dpca_decode",
     "dpca_copy_packed", "dpca_pass", "dpca_run", "dpca_crit_history", "dpca_write_outputs",
-    "dpca_orthonormalize", "dpca_syev", "dpca_close", "dpca_last_error",
+    "dpca_orthonormalize", "dpca_syev", "dpca_bench_pass", "dpca_close", "dpca_last_error",
 ]
 
 _lib = None
@@ -96,6 +96,7 @@ def lib():
     L.dpca_write_outputs.argtypes = [vp, ctypes.c_char_p, vp, vp]
     L.dpca_orthonormalize.argtypes = [vp, i64, i32, vp, P(i32), i32]
     L.dpca_syev.argtypes = [vp, i32, vp, vp, i32]
+    L.dpca_bench_pass.argtypes = [vp, vp, i32, vp]
     L.dpca_close.argtypes = [vp]
     L.dpca_close.restype = None
     _lib = L
@@ -241,6 +242,13 @@ class PCA:
         rep = Report()
         _check(lib().dpca_run(self._h, _ptr(rhs), _ptr(sigma), ctypes.byref(rep)))
         return sigma, rhs, rep.as_dict()
+
+    def bench_pass(self, Q: np.ndarray, npass: int):
+        """Event-timed passes: (ms per pass, ms Z Q stage, ms Z^T T stage, ms other)."""
+        Q = np.ascontiguousarray(Q, dtype=np.float64)
+        ms = np.zeros(4, np.float64)
+        _check(lib().dpca_bench_pass(self._h, _ptr(Q), npass, _ptr(ms)))
+        return ms
 
     def crit_history(self):
         buf = np.empty(1024, np.float64)

@@ -160,6 +160,11 @@ struct dpca_ctx {
     double *gpart = nullptr, *G = nullptr, *Rinv = nullptr, *w = nullptr, *X = nullptr,
            *dots = nullptr;
     int ngparts = 256;
+    // mixed form
+    uint4 *ztab = nullptr, *QB = nullptr, *UWB = nullptr;
+    float *Tpart = nullptr, *Apart = nullptr;
+    int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 8;
+    int64_t seg_bytes = 0, rps = 0;
     int *flags = nullptr;             // [0] shifted used, [1] QR failure, [2] Jacobi
     IterState *its = nullptr;
     std::vector<double> crit_hist;
@@ -232,7 +237,8 @@ static void free_ctx(dpca_ctx *c)
     (void)hipSetDevice(c->dev);
     void *ptrs[] = {c->packed, c->zval, c->mu, c->dsc, c->ssum, c->sctr, c->Q, c->ARHS,
                     c->RHS2, c->RHS2old, c->tmp, c->T, c->part, c->gpart, c->G, c->Rinv,
-                    c->w, c->X, c->dots, c->flags, c->its};
+                    c->w, c->X, c->dots, c->flags, c->its, c->ztab, c->QB, c->UWB,
+                    c->Tpart, c->Apart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (auto &tm : c->pending) {
@@ -324,7 +330,29 @@ static int alloc_work(dpca_ctx *c)
     int64_t sp = (2048 + tiles - 1) / tiles;
     sp = std::max<int64_t>(1, std::min<int64_t>(sp, c->Npad / 256));
     c->splits = (int)sp;
-    r |= dalloc(&c->part, (size_t)c->splits * mk);
+    if (c->p.precision == DPCA_FP64) {
+        r |= dalloc(&c->part, (size_t)c->splits * mk);
+    } else {
+        // T = Z Q: sample segments of <= 16384 samples (FP32 accumulation
+        // length), 8 or 2 SNP tiles per wave depending on how many
+        // workgroups that leaves.
+        const int64_t nsegs = (c->ldp + 4095) / 4096;
+        c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 64);
+        c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
+        c->st_tiles = (c->Npad / 1024 * c->nseg >= 512) ? 8 : 2;
+        // Z^T T: SNP splits of <= 16384 rows, >= ~1024 workgroups
+        c->sa_tiles = (c->ldp >= 256 * 32) ? 8 : 2;
+        const int64_t tiles = c->ldp / (c->sa_tiles * 32);
+        int64_t ns = std::max<int64_t>((1024 + tiles - 1) / tiles, (c->Npad + 16383) / 16384);
+        ns = std::min<int64_t>(ns, c->Npad / 128);
+        c->rps = round_up((c->Npad + ns - 1) / ns, 128);
+        c->nsplit = (int)((c->Npad + c->rps - 1) / c->rps);
+        r |= dalloc(&c->ztab, (size_t)c->Npad * sizeof(uint4));
+        r |= dalloc(&c->QB, (size_t)c->Mpad / 8 * c->kp * 2 * sizeof(uint4));
+        r |= dalloc(&c->UWB, (size_t)c->Npad / 4 * c->kp * 2 * sizeof(uint4));
+        r |= dalloc(&c->Tpart, (size_t)c->nseg * c->Npad * c->kp * sizeof(float));
+        r |= dalloc(&c->Apart, (size_t)c->nsplit * c->Mpad * c->kp * sizeof(float));
+    }
     r |= dalloc(&c->gpart, (size_t)c->ngparts * 64 * 64 * sizeof(double));
     r |= dalloc(&c->G, 64 * 64 * sizeof(double));
     r |= dalloc(&c->Rinv, 64 * 64 * sizeof(double));
@@ -347,6 +375,7 @@ static int compute_stats(dpca_ctx *c)
     Stage s(c, T_LOAD);
     DPCA_CHECK(launch_snp_stats(c->st, c->packed, c->Npad, c->ldp, 1.0 / sqrt((double)c->N),
                                 c->ssum, c->sctr, c->zval, c->mu, c->dsc));
+    if (c->p.precision == DPCA_MIXED) DPCA_CHECK(launch_ztab(c->st, c->zval, c->Npad, c->ztab));
     return 0;
 }
 
@@ -372,8 +401,8 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out)
         delete c;
         return r;
     }
-    if (c->p.precision != DPCA_FP64) {
-        set_error("precision %d is not available in this build", c->p.precision);
+    if (c->p.precision != DPCA_FP64 && c->p.precision != DPCA_MIXED) {
+        set_error("unknown precision %d", c->p.precision);
         delete c;
         return -1;
     }
@@ -434,13 +463,65 @@ static int pass_fp64(dpca_ctx *c, const double *Qin, double *out)
         Stage s(c, T_MM_AT);
         DPCA_CHECK(launch_zt_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->T, c->kp,
                                  c->splits, c->part));
+    }
+    {
+        Stage s(c, T_MM);
         DPCA_CHECK(launch_reduce_parts(c->st, c->part, c->splits, c->Mpad * c->kp, out, 0));
     }
     c->passes++;
     return 0;
 }
 
-static int do_pass(dpca_ctx *c, const double *Qin, double *out) { return pass_fp64(c, Qin, out); }
+// Mixed form: T = Z Q in bf16 hi+lo (FP64 T), then Z^T T on the raw codes.
+static int half_pass_mixed(dpca_ctx *c, const double *Qin)
+{
+    {
+        Stage s(c, T_MM);
+        DPCA_CHECK(launch_qsplit(c->st, Qin, c->Mpad, c->kp, c->QB));
+    }
+    {
+        Stage s(c, T_MM_A);
+        DPCA_CHECK(launch_za_mx(c->st, c->packed, c->Npad, c->ldp, c->ztab, c->QB, c->kp,
+                                c->st_tiles, c->seg_bytes, c->nseg, c->Tpart));
+    }
+    {
+        Stage s(c, T_MM);
+        DPCA_CHECK(launch_tprep(c->st, c->Tpart, c->nseg, c->Npad, c->kp, c->mu, c->dsc, c->T,
+                                c->UWB));
+    }
+    return 0;
+}
+
+static int pass_mixed(dpca_ctx *c, const double *Qin, double *out)
+{
+    int r = half_pass_mixed(c, Qin);
+    if (r) return r;
+    {
+        Stage s(c, T_MM_AT);
+        DPCA_CHECK(launch_zt_mx(c->st, c->packed, c->Npad, c->ldp, c->UWB, c->kp, c->sa_tiles,
+                                c->rps, c->nsplit, c->Apart));
+    }
+    {
+        Stage s(c, T_MM);
+        DPCA_CHECK(launch_reduce_f32(c->st, c->Apart, c->nsplit, c->Mpad * c->kp, out));
+    }
+    c->passes++;
+    return 0;
+}
+
+static int do_pass(dpca_ctx *c, const double *Qin, double *out)
+{
+    return c->p.precision == DPCA_MIXED ? pass_mixed(c, Qin, out) : pass_fp64(c, Qin, out);
+}
+
+// Z Q only (the in-core variant's half pass), into c->T.
+static int half_pass(dpca_ctx *c, const double *Qin)
+{
+    if (c->p.precision == DPCA_MIXED) return half_pass_mixed(c, Qin);
+    Stage s(c, T_MM_A);
+    DPCA_CHECK(launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, Qin, c->kp, c->T));
+    return 0;
+}
 
 // CholQR2 (+ shifted fallback) of A (Mpad x kp) into c->Q; uses c->tmp.
 static int orthonormalise(dpca_ctx *c, const double *A)
@@ -512,11 +593,7 @@ static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
                 if ((r = do_pass(c, c->Q, c->ARHS))) return r;
             }
             if ((r = orthonormalise(c, c->ARHS))) return r;
-            {
-                Stage s(c, T_MM_A);
-                DPCA_CHECK(launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->Q, c->kp,
-                                         c->T));
-            }
+            if ((r = half_pass(c, c->Q))) return r;
             {
                 Stage s(c, T_RR);
                 DPCA_CHECK(launch_gram(c->st, c->T, c->T, c->Npad, c->kp, c->gpart, c->ngparts,
@@ -1119,6 +1196,45 @@ int dpca_syev(const double *B, int k, double *w, double *X, int device)
         set_error("Jacobi eigensolver did not converge");
         return -4;
     }
+    return 0;
+}
+
+int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
+{
+    if (!c || !Q || !ms || npass < 1) {
+        set_error("dpca_bench_pass: bad arguments");
+        return -1;
+    }
+    DPCA_CHECK(hipSetDevice(c->dev));
+    int r;
+    if ((r = copy_in_rhs(c, Q, c->RHS2))) return r;
+    DPCA_CHECK(hipStreamSynchronize(c->st));
+    collect_timers(c);
+    for (int i = 0; i < T_NKIND; i++) c->t[i] = 0;
+    hipEvent_t e0 = get_event(c), e1 = get_event(c);
+    DPCA_CHECK(hipEventRecord(e0, c->st));
+    for (int i = 0; i < npass; i++)
+        if ((r = do_pass(c, c->RHS2, c->ARHS))) return r;
+    DPCA_CHECK(hipEventRecord(e1, c->st));
+    DPCA_CHECK(hipStreamSynchronize(c->st));
+    float tot = 0;
+    DPCA_CHECK(hipEventElapsedTime(&tot, e0, e1));
+    c->free_ev.push_back(e0);
+    c->free_ev.push_back(e1);
+    // per-kernel times: the stage events bracket the kernels of each stage
+    double ka = 0, kb = 0;
+    for (auto &tm : c->pending) {
+        float t = 0;
+        if (tm.a && tm.b && hipEventElapsedTime(&t, tm.a, tm.b) == hipSuccess) {
+            if (tm.kind == T_MM_A) ka += t;
+            if (tm.kind == T_MM_AT) kb += t;
+        }
+    }
+    collect_timers(c);
+    ms[0] = tot / npass;
+    ms[1] = ka / npass;
+    ms[2] = kb / npass;
+    ms[3] = ms[0] - ms[1] - ms[2];
     return 0;
 }
 

@@ -21,11 +21,11 @@ void set_error(const char *fmt, ...);
 
 // ---------------------------------------------------------------- layout --
 // Packed genotypes live in HBM SNP-major like the .bed file, but each row is
-// padded to `ldp` bytes (a multiple of 64, i.e. 256 samples) and the row count
-// to a multiple of 256.  Padding codes are 01 (missing), so they decode to 0
+// padded to `ldp` bytes (a multiple of 256, i.e. 1024 samples) and the row
+// count to a multiple of 1024.  Padding codes are 01 (missing), so they decode to 0
 // in both numerical forms and no kernel needs a bounds check.
-constexpr int kRowAlign = 256;    // SNP rows padded to this multiple
-constexpr int kLdpAlign = 64;     // bytes per padded row: multiple of this
+constexpr int kRowAlign = 1024;   // SNP rows padded to this multiple
+constexpr int kLdpAlign = 256;    // bytes per padded row: multiple of this
 constexpr uint8_t kPadByte = 0x55;
 
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
@@ -62,16 +62,19 @@ hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, in
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
                                double *out, int accumulate);
 
-// Mixed form (bf16 hi+lo operands, FP32 accumulate).
-hipError_t launch_split_bf16(hipStream_t st, const double *Q, int64_t rows, int kp,
-                             const double *rowscale, uint32_t *hl);
-hipError_t launch_za_bf16(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                          const uint32_t *Qhl, int kp, float *HQ);
-hipError_t launch_zt_bf16(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                          const uint32_t *Uhl, int kp, int splits, float *part);
-hipError_t launch_mask_fix_a(hipStream_t st, const float *HQ, const double *mu,
-                             const double *dsc, const double *colsumQ,
-                             const float *missQ, int64_t rows, int kp, double *T);
+// Mixed form (bf16 hi+lo operands, FP32 accumulate); see k_mixed.hip.
+hipError_t launch_ztab(hipStream_t st, const double *zval, int64_t rows, uint4 *ztab);
+hipError_t launch_qsplit(hipStream_t st, const double *Q, int64_t Mpad, int kp, uint4 *QB);
+hipError_t launch_za_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                        const uint4 *ztab, const uint4 *QB, int kp, int st_tiles,
+                        int64_t seg_bytes, int nseg, float *Tpart);
+hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t rows, int kp,
+                        const double *mu, const double *dsc, double *T, uint4 *UWB);
+hipError_t launch_zt_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                        const uint4 *UWB, int kp, int sa_tiles, int64_t rows_per_split,
+                        int nsplit, float *part);
+hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
+                             double *out);
 
 // Tall-skinny helpers, FP64.  X, Y: m x kp row-major.
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,

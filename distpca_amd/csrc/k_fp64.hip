@@ -166,7 +166,8 @@ hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t ro
 
 // ---------------------------------------------------------- FP64  T = Z Q --
 // v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16).  Lane l = 16*g + r
-// supplies A[r][g] and B[g][r] and holds D[4g + q][r], q = 0..3.
+// supplies A[r][g] and B[g][r] and holds D[4q + g][r], q = 0..3 (rows
+// interleaved by lane group; checked on the device).
 // Rows of the tile are SNPs, columns are right-hand sides, and the 4-deep K
 // slot g is a sample.  Which sample sits in which K slot is free as long as
 // A and B agree: lane group g walks the 64 samples of its own 16 packed
@@ -221,8 +222,8 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
     for (int c = 0; c < CT; c++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            T[(base + 4 * g + q) * KP + c * 16 + r] = acc0[c][q];
-            T[(base + 16 + 4 * g + q) * KP + c * 16 + r] = acc1[c][q];
+            T[(base + 4 * q + g) * KP + c * 16 + r] = acc0[c][q];
+            T[(base + 16 + 4 * q + g) * KP + c * 16 + r] = acc1[c][q];
         }
 }
 
@@ -303,7 +304,7 @@ __global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ pack
 #pragma unroll
         for (int c = 0; c < CT; c++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) out[(a * 16 + 4 * g + q) * KP + c * 16 + r] = acc[a][c][q];
+            for (int q = 0; q < 4; q++) out[(a * 16 + 4 * q + g) * KP + c * 16 + r] = acc[a][c][q];
 }
 
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,

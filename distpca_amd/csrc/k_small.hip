@@ -90,15 +90,20 @@ hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t
 //   mode 0  factor G; if a pivot is not positive, raise the diagonal by
 //           shift_mult * trace(G), factor again and set flags[0] = 1;
 //   mode 1  factor G;
-//   mode 2  third pass: R^{-1} = I unless flags[0] says the shift was used.
+//   mode 2  third pass: R^{-1} = I unless the shift was used or the Gram of
+//           the CholQR2 result is not yet the identity.
 // A failure that survives the shift sets flags[1] = 1.
-__device__ int chol_factor(double (*A)[kMaxKp + 1], int k, int *bad)
+// Right-looking Cholesky of the leading k x k block of A (upper triangle).
+// A pivot that is not positive, or that has lost all but ~1e-14 of its
+// column's diagonal (the column lies within ~1e-7 rad of the span of the
+// previous ones, beyond what CholQR2 can orthonormalise), marks the factor bad.
+__device__ int chol_factor(double (*A)[kMaxKp + 1], const double *diag0, int k, int *bad)
 {
     const int tid = threadIdx.x;
     for (int j = 0; j < k; j++) {
         if (tid == 0) {
             const double d = A[j][j];
-            if (!(d > 0.0)) *bad = 1;
+            if (!(d > 0.0) || !(d > 1e-14 * diag0[j])) *bad = 1;
             A[j][j] = sqrt(d > 0.0 ? d : 1.0);
         }
         __syncthreads();
@@ -115,17 +120,34 @@ __device__ int chol_factor(double (*A)[kMaxKp + 1], int k, int *bad)
     return *bad;
 }
 
+// mode 2 also runs a real third pass when the Gram of the CholQR2 result is
+// not the identity to 1e-12 (flags[3] records it).
 __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, int k,
                                                   int kp, double shift_mult, int mode,
                                                   double *__restrict__ Rinv, int *flags)
 {
     __shared__ double A[kMaxKp][kMaxKp + 1];
     __shared__ double col[kMaxKp];
+    __shared__ double diag0[kMaxKp];
     __shared__ int bad, skip;
     const int tid = threadIdx.x;
     if (tid == 0) {
         bad = 0;
-        skip = (mode == 2 && flags[0] == 0);
+        skip = 0;
+    }
+    for (int e = tid; e < kp * kp; e += 256) {
+        const int i = e / kp, j = e % kp;
+        A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
+    }
+    __syncthreads();
+    // the relative-pivot test applies to the first factorisation only
+    if (tid < k) diag0[tid] = (mode == 0) ? A[tid][tid] : 0.0;
+    if (mode == 2 && tid == 0 && flags[0] == 0) {
+        double dev = 0;
+        for (int i = 0; i < k; i++)
+            for (int j = i; j < k; j++) dev = fmax(dev, fabs(A[i][j] - (i == j ? 1.0 : 0.0)));
+        skip = !(dev > 1e-12);
+        if (!skip) flags[3] = 1;
     }
     __syncthreads();
     if (skip) {
@@ -133,12 +155,7 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
             Rinv[e] = (e / kp == e % kp && e / kp < k) ? 1.0 : 0.0;
         return;
     }
-    for (int e = tid; e < kp * kp; e += 256) {
-        const int i = e / kp, j = e % kp;
-        A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
-    }
-    __syncthreads();
-    chol_factor(A, k, &bad);
+    chol_factor(A, diag0, k, &bad);
     if (bad && mode == 0) {
         __syncthreads();
         for (int e = tid; e < kp * kp; e += 256) {
@@ -150,12 +167,15 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
             double tr = 0;
             for (int i = 0; i < k; i++) tr += A[i][i];
             const double sh = shift_mult * tr;
-            for (int i = 0; i < k; i++) A[i][i] += sh;
+            for (int i = 0; i < k; i++) {
+                A[i][i] += sh;
+                diag0[i] = 0.0;  // the shifted factor only needs positive pivots
+            }
             bad = 0;
             flags[0] = 1;
         }
         __syncthreads();
-        chol_factor(A, k, &bad);
+        chol_factor(A, diag0, k, &bad);
     }
     // in-place inverse of the upper triangle, one column at a time
     for (int j = 0; j < k; j++) {

@@ -148,6 +148,13 @@ int dpca_orthonormalize(const double *A, long long m, int k, double *Q, int *shi
  * eigenvalues descending in w, unit eigenvectors as the columns of X. */
 int dpca_syev(const double *B, int k, double *w, double *X, int device);
 
+/* Measurement hook (no reference counterpart): runs npass passes of the
+ * configured form on Q (M x nrhs) and returns event-timed averages on the
+ * library's stream, in milliseconds: ms[0] whole pass, ms[1] the Z Q kernel,
+ * ms[2] the Z^T T kernel, ms[3] everything else in the pass (operand
+ * splits, partial reductions). */
+int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms);
+
 void dpca_close(dpca_ctx *c);
 const char *dpca_last_error(void);
 

@@ -5,6 +5,10 @@ that has the reference tree; the outputs are committed).
       the reference's own example genotypes (example/ of the reference),
       942 samples x 133,594 SNPs; the .bim is regenerated by the tests (only
       its line count is read).
+  hgdp_golden.npz, c1_golden.npz also hold alt_* results: the same oracle run
+      with bsize 1000, i.e. only the summation order changed; the GPU parity
+      tests measure their tolerance for unconverged components against that
+      spread.
   hgdp_golden.npz
       oracle run of the reference's example settings (example/example.sh:5-12:
       nsv 20, nrhs 40, crit 2, tol 1e-3, bsize 100, power 1): sigma, Ritz
@@ -43,9 +47,14 @@ def hgdp():
     p = O.resolve(O.Params(nsv=20, nrhs=40, crit=2, tol=1e-3, bsize=100), M, N)
     r = O.rsi(packed, M, N, p)
     s, c, a, d, z = O.snp_stats(packed, N, M)
+    # the same run with another block size: how far the reference's own result
+    # moves when only its summation order changes
+    pa = O.resolve(O.Params(nsv=20, nrhs=40, crit=2, tol=1e-3, bsize=1000), M, N)
+    ra = O.rsi(packed, M, N, pa)
     np.savez_compressed(os.path.join(HERE, "hgdp_golden.npz"), sigma=r.sigma, rhs=r.rhs,
                         crit=np.array(r.crit_values), total_its=r.total_its,
-                        passes=r.passes, sum=s, ctr=c, zval=z)
+                        passes=r.passes, sum=s, ctr=c, zval=z, alt_sigma=ra.sigma,
+                        alt_rhs=ra.rhs, alt_crit=np.array(ra.crit_values))
     print("hgdp", r.total_its, r.sigma[:4])
 
 
@@ -59,7 +68,11 @@ def c1():
                            bsize=4096), M, N)
     r = O.rsi(packed, M, N, p)
     s, c, a, dd, z = O.snp_stats(packed, N, M)
+    pa = O.resolve(O.Params(nsv=10, nrhs=20, power=4, crit=1, tol=1e-6, miter=20,
+                            bsize=1000), M, N)
+    ra = O.rsi(packed, M, N, pa)
     np.savez_compressed(os.path.join(HERE, "c1_golden.npz"), sigma=r.sigma, rhs=r.rhs,
+                        alt_sigma=ra.sigma, alt_rhs=ra.rhs,
                         crit=np.array(r.crit_values), total_its=r.total_its,
                         passes=r.passes, sum=s, ctr=c, zval=z,
                         bed_sha=np.frombuffer(__import__("hashlib").sha256(

@@ -2,8 +2,11 @@
 
 Tolerances (FP64 form vs the reference's CPU arithmetic, restated by the
 oracle):  decode and per-SNP statistics bit-exact; one pass within 1e-12 of
-the pass norm; eigenvalues (sigma^2) within 1e-10 relative; principal angles
-of the top-nsv subspace within 1e-8 rad where the iteration has converged.
+the pass norm.  For whole runs: eigenvalues (sigma^2) within 1e-10 relative
+and principal angles of every top-j subspace within 1e-8 rad -- or, for the
+components the iteration has not converged (the noise bulk), within 20x of
+how far the oracle itself moves when only its summation order changes
+(the alt_* runs of the golden fixtures, bsize 1000 instead of 100/4096).
 """
 import os
 import subprocess
@@ -103,8 +106,11 @@ def test_orthonormalize_cholqr2():
     R = Q.T @ A
     assert np.abs(np.tril(R, -1)).max() < 1e-9 * np.abs(R).max()
     assert np.all(np.diag(R) > 0)
-    # ill-conditioned input: the shifted fallback keeps Q orthonormal
-    B = rng.standard_normal((4000, 30)) @ np.diag(np.logspace(0, 12, 30))
+    # nearly collinear columns (kappa 1e12): the shifted fallback keeps Q
+    # orthonormal and spanning the same space
+    U = np.linalg.qr(rng.standard_normal((4000, 30)))[0]
+    V = np.linalg.qr(rng.standard_normal((30, 30)))[0]
+    B = U @ np.diag(np.logspace(0, -12, 30)) @ V.T
     Q2, shifted2 = D.orthonormalize(B)
     assert shifted2
     assert np.abs(Q2.T @ Q2 - np.eye(30)).max() < 1e-12
@@ -125,6 +131,21 @@ def test_syev_jacobi(k):
     assert np.abs(S @ X - X * w).max() < 1e-11 * ref[0]
 
 
+def check_run(sigma, rhs, g, nsv, fac=20.0):
+    """Per-component parity against the golden oracle run (see module doc)."""
+    ev, ev_alt = g["sigma"] ** 2, g["alt_sigma"] ** 2
+    spread = np.abs(ev_alt - ev) / ev
+    rel = np.abs(sigma ** 2 - ev) / ev
+    tol_e = np.maximum(1e-10, fac * spread)
+    print("eig rel err", rel, "\n  oracle spread", spread)
+    assert np.all(rel <= tol_e), (rel, tol_e)
+    for j in range(1, nsv + 1):
+        a = O.principal_angles(rhs[:, :j], g["rhs"][:, :j]).max()
+        sp = O.principal_angles(g["alt_rhs"][:, :j], g["rhs"][:, :j]).max()
+        assert a <= max(1e-8, fac * sp), (j, a, sp)
+    return rel
+
+
 def test_run_hgdp_matches_reference_settings(hgdp_prefix, hgdp_golden):
     # example/example.sh:5-12 settings; the published log implies 11 iterations
     _banner("run HGDP nsv 20 nrhs 40 crit 2 tol 1e-3 bsize 100")
@@ -133,13 +154,11 @@ def test_run_hgdp_matches_reference_settings(hgdp_prefix, hgdp_golden):
         sigma, rhs, rep = pca.run()
         crit = pca.crit_history()
     assert rep["total_its"] == 11 and rep["passes"] == 22
-    assert np.allclose(crit, hgdp_golden["crit"], rtol=1e-6, atol=1e-12)
-    rel = np.abs(sigma ** 2 - hgdp_golden["sigma"] ** 2) / hgdp_golden["sigma"] ** 2
-    print("HGDP eigenvalue rel err max", rel.max())
-    assert rel.max() < 1e-10
-    ang = O.principal_angles(rhs[:, :20], hgdp_golden["rhs"][:, :20])
-    print("HGDP principal angles max", ang.max())
-    assert ang.max() < 1e-8
+    # the MEV criterion follows the noise components; the oracle's own
+    # trajectory moves by ~1.4e-4 relative when only bsize changes
+    assert np.allclose(crit, hgdp_golden["crit"], rtol=2e-3)
+    rel = check_run(sigma, rhs, hgdp_golden, 20)
+    assert rel[:10].max() < 1e-10
 
 
 def test_run_c1_synthetic(c1_golden):
@@ -152,12 +171,10 @@ def test_run_c1_synthetic(c1_golden):
         assert np.array_equal(sha, c1_golden["bed_sha"]), "device generator != host generator"
         sigma, rhs, rep = pca.run()
     assert rep["total_its"] == int(c1_golden["total_its"])
-    rel = np.abs(sigma ** 2 - c1_golden["sigma"] ** 2) / c1_golden["sigma"] ** 2
-    ang = O.principal_angles(rhs[:, :10], c1_golden["rhs"][:, :10])
-    print("c1 eigenvalue rel err", rel.max(), "angles", ang)
-    assert rel.max() < 1e-10
     # PCs 1-2 carry the 3-population structure and converge; PCs 3-10 sit in
     # the noise bulk and are not converged after 20 iterations
+    rel = check_run(sigma, rhs, c1_golden, 10)
+    assert rel[:2].max() < 1e-10
     assert O.principal_angles(rhs[:, :2], c1_golden["rhs"][:, :2]).max() < 1e-8
 
 
@@ -191,9 +208,17 @@ def test_cli_dropin(tmp_path):
     M, N, iids, _, packed = O.read_plink(pre)
     r = O.rsi(packed, M, N, O.resolve(O.Params(nsv=4, nrhs=8, power=2, crit=2, tol=1e-6,
                                                bsize=500), M, N))
-    assert np.allclose(sv, r.sigma, rtol=1e-10)
+    ra = O.rsi(packed, M, N, O.resolve(O.Params(nsv=4, nrhs=8, power=2, crit=2, tol=1e-6,
+                                                bsize=4000), M, N))
     txt = open(str(tmp_path / "res_leftSingularVectors.txt")).read().splitlines()
     assert txt[0] == "FID\tPC0\tPC1\tPC2\tPC3"
     assert txt[1].split("\t")[0].strip() == iids[0] and len(txt) == M + 1
     U = np.array([[float(x) for x in l.split("\t")[1:]] for l in txt[1:]])
-    assert O.principal_angles(U, r.rhs[:, :4]).max() < 1e-8
+    # the files carry 13 decimals (src/methods.cpp:1093,1114)
+    rel = np.abs(sv ** 2 - r.sigma ** 2) / r.sigma ** 2
+    spread = np.abs(ra.sigma ** 2 - r.sigma ** 2) / r.sigma ** 2
+    assert np.all(rel <= np.maximum(1e-10, 20 * spread)), (rel, spread)
+    for j in range(1, 5):
+        a = O.principal_angles(U[:, :j], r.rhs[:, :j]).max()
+        sp = O.principal_angles(ra.rhs[:, :j], r.rhs[:, :j]).max()
+        assert a <= max(1e-8, 20 * sp), (j, a, sp)
