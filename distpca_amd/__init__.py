@@ -244,9 +244,10 @@ class PCA:
         return sigma, rhs, rep.as_dict()
 
     def bench_pass(self, Q: np.ndarray, npass: int):
-        """Event-timed passes: (ms per pass, ms Z Q stage, ms Z^T T stage, ms other)."""
+        """Event-timed passes: (ms per pass, ms Z Q kernel, ms Z^T T kernel, ms other,
+        host wall ms per pass)."""
         Q = np.ascontiguousarray(Q, dtype=np.float64)
-        ms = np.zeros(4, np.float64)
+        ms = np.zeros(5, np.float64)
         _check(lib().dpca_bench_pass(self._h, _ptr(Q), npass, _ptr(ms)))
         return ms
 

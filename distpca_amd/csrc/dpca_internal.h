@@ -39,6 +39,11 @@ hipError_t launch_repack(hipStream_t st, const uint8_t *raw, int64_t rows, int64
                          uint8_t *dst, int64_t ldp, int M);
 hipError_t launch_fill_u8(hipStream_t st, uint8_t *p, int64_t n, uint8_t v);
 hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v);
+// M x k dense <-> Mpad x kp zero-padded right-hand sides
+hipError_t launch_pad_rhs(hipStream_t st, const double *src, int M, int k, int64_t Mpad, int kp,
+                          double *dst);
+hipError_t launch_compact_rhs(hipStream_t st, const double *src, int M, int k, int kp,
+                              double *dst);
 
 // Per-SNP integer statistics and the FP64 value table zval[4*s + code].
 // Also the mixed-form per-SNP mean mu[s] and scale dsc[s] = 1/(sd*sqrt(N))

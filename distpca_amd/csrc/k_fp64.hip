@@ -71,6 +71,39 @@ hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v)
     return hipGetLastError();
 }
 
+// M x k (dense) <-> Mpad x kp (zero padded) for the right-hand sides.
+__global__ void k_pad_rhs(const double *__restrict__ src, int M, int k, int64_t Mpad, int kp,
+                          double *__restrict__ dst)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= Mpad * kp) return;
+    const int64_t i = e / kp;
+    const int j = (int)(e % kp);
+    dst[e] = (i < M && j < k) ? src[i * k + j] : 0.0;
+}
+
+__global__ void k_compact_rhs(const double *__restrict__ src, int M, int k, int kp,
+                              double *__restrict__ dst)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)M * k) return;
+    dst[e] = src[(e / k) * kp + e % k];
+}
+
+hipError_t launch_pad_rhs(hipStream_t st, const double *src, int M, int k, int64_t Mpad, int kp,
+                          double *dst)
+{
+    k_pad_rhs<<<(unsigned)((Mpad * kp + 255) / 256), 256, 0, st>>>(src, M, k, Mpad, kp, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_compact_rhs(hipStream_t st, const double *src, int M, int k, int kp,
+                              double *dst)
+{
+    k_compact_rhs<<<(unsigned)(((int64_t)M * k + 255) / 256), 256, 0, st>>>(src, M, k, kp, dst);
+    return hipGetLastError();
+}
+
 // --------------------------------------------------------------- SNP stats --
 // One wavefront per SNP row.  Codes are counted 16 at a time with popcounts:
 // with lo = bit0 and hi = bit1 of every 2-bit field, missing (01) = lo & ~hi,

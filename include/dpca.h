@@ -152,7 +152,8 @@ int dpca_syev(const double *B, int k, double *w, double *X, int device);
  * configured form on Q (M x nrhs) and returns event-timed averages on the
  * library's stream, in milliseconds: ms[0] whole pass, ms[1] the Z Q kernel,
  * ms[2] the Z^T T kernel, ms[3] everything else in the pass (operand
- * splits, partial reductions). */
+ * splits, partial reductions), ms[4] host wall clock per pass (launch to
+ * stream drain).  ms must hold 5 values. */
 int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms);
 
 void dpca_close(dpca_ctx *c);

@@ -114,8 +114,10 @@ def test_orthonormalize_cholqr2():
     Q2, shifted2 = D.orthonormalize(B)
     assert shifted2
     assert np.abs(Q2.T @ Q2 - np.eye(30)).max() < 1e-12
+    # the span is only determined to ~kappa * eps in its weakest directions;
+    # Householder QR (numpy) is the yardstick for that loss
     qn, _ = np.linalg.qr(B)
-    assert O.principal_angles(qn, Q2).max() < 1e-6
+    assert O.principal_angles(U, Q2).max() <= 10 * O.principal_angles(U, qn).max() + 1e-12
 
 
 @pytest.mark.parametrize("k", [1, 7, 20, 40, 64])
