@@ -339,7 +339,8 @@ static int alloc_work(dpca_ctx *c)
         const int64_t nsegs = (c->ldp + 4095) / 4096;
         c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 64);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
-        c->st_tiles = (c->Npad / 1024 * c->nseg >= 512) ? 8 : 2;
+        c->st_tiles = (c->Npad / 512 * c->nseg >= 512) ? 4 : 2;
+        if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 2, 4, 8
         // Z^T T: SNP splits of <= 16384 rows, >= ~1024 workgroups
         c->sa_tiles = (c->ldp >= 256 * 32) ? 8 : 2;
         const int64_t tiles = c->ldp / (c->sa_tiles * 32);

@@ -74,7 +74,9 @@ hipError_t launch_ztab(hipStream_t st, const double *zval, int64_t rows, uint4 *
 }
 
 // --------------------------------------------------------------- Q split ----
-// QB[(i/8)*kp + c] = {8 bf16 hi of Q[i..i+7][c]}, {8 bf16 lo}  (two uint4)
+// QB[((i/8)*2 + 0)*kp + c] = 8 bf16 hi of Q[i..i+7][c], [((i/8)*2 + 1)*kp + c]
+// = the 8 lo parts: hi and lo rows apart, so 16 lanes reading 16 columns
+// touch 256 contiguous bytes of LDS (no bank conflicts).
 __global__ void k_qsplit(const double *__restrict__ Q, int64_t groups, int kp,
                          uint4 *__restrict__ QB)
 {
@@ -85,10 +87,10 @@ __global__ void k_qsplit(const double *__restrict__ Q, int64_t groups, int kp,
     uint32_t h[8], l[8];
 #pragma unroll
     for (int e = 0; e < 8; e++) split_bf16(Q[(g8 * 8 + e) * kp + c], h[e], l[e]);
-    QB[2 * idx] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16),
-                             h[6] | (h[7] << 16));
-    QB[2 * idx + 1] = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16),
-                                 l[6] | (l[7] << 16));
+    QB[(g8 * 2) * kp + c] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16),
+                                       h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+    QB[(g8 * 2 + 1) * kp + c] = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16),
+                                           l[4] | (l[5] << 16), l[6] | (l[7] << 16));
 }
 
 hipError_t launch_qsplit(hipStream_t st, const double *Q, int64_t Mpad, int kp, uint4 *QB)
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(512) void k_za_mx(const uint8_t *__restrict__ packe
                                                float *__restrict__ Tpart)
 {
     constexpr int KP = CT * 16;
-    __shared__ uint4 qs[32 * KP * 2];  // [group of 8 samples][col][hi, lo]
+    __shared__ u32x4 qs[32 * KP * 2];  // [group of 8 samples][hi, lo][col]
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
     const int64_t base = (int64_t)blockIdx.x * (8 * ST * 16) + wave * (ST * 16);
@@ -119,40 +121,53 @@ __global__ __launch_bounds__(512) void k_za_mx(const uint8_t *__restrict__ packe
     int64_t b1 = b0 + seg_bytes;
     if (b1 > ldp) b1 = ldp;
 
-    uint4 tab[ST];
+    u32x4 tab[ST];
     f4 acc[ST][CT];
 #pragma unroll
     for (int s = 0; s < ST; s++) {
-        tab[s] = ztab[base + s * 16 + r];
+        tab[s] = reinterpret_cast<const u32x4 *>(ztab)[base + s * 16 + r];
 #pragma unroll
         for (int c = 0; c < CT; c++) acc[s][c] = f4{0, 0, 0, 0};
     }
 
+    // Software pipeline: the next chunk's Q operand and packed bytes are
+    // loaded into registers while the current chunk is multiplied.
+    constexpr int NQ = 32 * KP * 2 / 512;
+    static_assert(32 * KP * 2 % 512 == 0, "tile");
+    u32x4 rq[NQ], rp[ST];
+#define DPCA_ZA_LOAD(c0)                                                                    \
+    do {                                                                                    \
+        const u32x4 *src_ = reinterpret_cast<const u32x4 *>(QB) + ((c0) / 2) * KP * 2;      \
+        _Pragma("unroll") for (int u = 0; u < NQ; u++) rq[u] = src_[tid + 512 * u];        \
+        _Pragma("unroll") for (int s = 0; s < ST; s++) rp[s] =                             \
+            *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + (c0) +     \
+                                             16 * g);                                       \
+    } while (0)
+    DPCA_ZA_LOAD(b0 < b1 ? b0 : 0);
+
     for (int64_t cb = b0; cb < b1; cb += 64) {
         __syncthreads();
-        {
-            const uint4 *src = QB + (cb / 2) * KP * 2;  // 4*cb samples = cb/2 groups
-            for (int e = tid; e < 32 * KP * 2; e += 512) qs[e] = src[e];
-        }
-        uint4 pk[ST];
 #pragma unroll
-        for (int s = 0; s < ST; s++)
-            pk[s] = *reinterpret_cast<const uint4 *>(packed + (base + s * 16 + r) * ldp + cb +
-                                                     16 * g);
+        for (int u = 0; u < NQ; u++) qs[tid + 512 * u] = rq[u];
+        u32x4 pk[ST];
+#pragma unroll
+        for (int s = 0; s < ST; s++) pk[s] = rp[s];
         __syncthreads();
+        {
+            const int64_t nx = (cb + 64 < b1) ? cb + 64 : cb;
+            DPCA_ZA_LOAD(nx);
+        }
 #pragma unroll 2
         for (int t = 0; t < 8; t++) {
             u32x4 bh[CT], bl[CT];
 #pragma unroll
             for (int c = 0; c < CT; c++) {
-                const uint4 h = qs[((8 * g + t) * KP + c * 16 + r) * 2];
-                const uint4 l = qs[((8 * g + t) * KP + c * 16 + r) * 2 + 1];
-                bh[c] = u32x4{h.x, h.y, h.z, h.w};
-                bl[c] = u32x4{l.x, l.y, l.z, l.w};
+                bh[c] = qs[((8 * g + t) * 2) * KP + c * 16 + r];
+                bl[c] = qs[((8 * g + t) * 2 + 1) * KP + c * 16 + r];
             }
 #pragma unroll
             for (int s = 0; s < ST; s++) {
-                const uint32_t w = (t < 2) ? pk[s].x : (t < 4) ? pk[s].y : (t < 6) ? pk[s].z : pk[s].w;
+                const uint32_t w = pk[s][t >> 1];
                 const uint32_t x16 = (t & 1) ? (w >> 16) : (w & 0xFFFFu);
                 const uint32_t c03 = spread4(x16 & 0xFFu), c47 = spread4(x16 >> 8);
                 uint32_t sel[4];
@@ -163,8 +178,8 @@ __global__ __launch_bounds__(512) void k_za_mx(const uint8_t *__restrict__ packe
                 u32x4 ah, al;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
-                    ah[j] = __builtin_amdgcn_perm(tab[s].y, tab[s].x, sel[j]);
-                    al[j] = __builtin_amdgcn_perm(tab[s].w, tab[s].z, sel[j]);
+                    ah[j] = __builtin_amdgcn_perm(tab[s][1], tab[s][0], sel[j]);
+                    al[j] = __builtin_amdgcn_perm(tab[s][3], tab[s][2], sel[j]);
                 }
 #pragma unroll
                 for (int c = 0; c < CT; c++) {
@@ -210,13 +225,15 @@ hipError_t launch_za_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int
     if (rows % per) return hipErrorInvalidValue;
     dim3 grid((unsigned)(rows / per), (unsigned)nseg);
     if (st_tiles == 8) return za_mx_ct<8>(st, grid, packed, ldp, ztab, QB, kp, seg_bytes, rows, Tpart);
+    if (st_tiles == 4) return za_mx_ct<4>(st, grid, packed, ldp, ztab, QB, kp, seg_bytes, rows, Tpart);
     if (st_tiles == 2) return za_mx_ct<2>(st, grid, packed, ldp, ztab, QB, kp, seg_bytes, rows, Tpart);
     return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------ T segments -> T, U, W -----
 // T[s][c] = sum of the segment partials (FP64); U = d_s T, W = -d_s mu_s T;
-// UWB[(s/4)*kp + c] = {bf16 hi of U(s0..s3), W(s0..s3)}, {bf16 lo ...}.
+// UWB[((s/4)*2 + 0)*kp + c] = bf16 hi of U(s0..s3), W(s0..s3);
+// UWB[((s/4)*2 + 1)*kp + c] = the lo parts.
 __global__ void k_tprep(const float *__restrict__ Tpart, int nseg, int64_t rows, int kp,
                         const double *__restrict__ mu, const double *__restrict__ dsc,
                         double *__restrict__ T, uint4 *__restrict__ UWB)
@@ -237,10 +254,10 @@ __global__ void k_tprep(const float *__restrict__ Tpart, int nseg, int64_t rows,
         split_bf16(u, uh[e], ul[e]);
         split_bf16(w, wh[e], wl[e]);
     }
-    UWB[2 * idx] = make_uint4(uh[0] | (uh[1] << 16), uh[2] | (uh[3] << 16),
-                              wh[0] | (wh[1] << 16), wh[2] | (wh[3] << 16));
-    UWB[2 * idx + 1] = make_uint4(ul[0] | (ul[1] << 16), ul[2] | (ul[3] << 16),
-                                  wl[0] | (wl[1] << 16), wl[2] | (wl[3] << 16));
+    UWB[(s4 * 2) * kp + c] = make_uint4(uh[0] | (uh[1] << 16), uh[2] | (uh[3] << 16),
+                                        wh[0] | (wh[1] << 16), wh[2] | (wh[3] << 16));
+    UWB[(s4 * 2 + 1) * kp + c] = make_uint4(ul[0] | (ul[1] << 16), ul[2] | (ul[3] << 16),
+                                            wl[0] | (wl[1] << 16), wl[2] | (wl[3] << 16));
 }
 
 hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t rows, int kp,
@@ -268,7 +285,7 @@ __global__ __launch_bounds__(512) void k_zt_mx(const uint8_t *__restrict__ packe
     constexpr int TB = SA * 16 * 8 / 4;    // packed bytes per row in this tile
     constexpr int SC = 128;                // SNPs per chunk
     __shared__ uint32_t pk[TB][SC / 4];    // [byte column][4-SNP group]
-    __shared__ uint4 uw[(SC / 4) * KP * 2];
+    __shared__ u32x4 uw[(SC / 4) * KP * 2];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
     const int64_t mb = (int64_t)blockIdx.x * TB;
@@ -284,20 +301,37 @@ __global__ __launch_bounds__(512) void k_zt_mx(const uint8_t *__restrict__ packe
 #pragma unroll
         for (int c = 0; c < CT; c++) acc[a][c] = f4{0, 0, 0, 0};
 
+    // Software pipeline: the next chunk's packed rows and U/W operands are
+    // loaded into registers while the current chunk is multiplied out of LDS.
+    constexpr int NPK = (SC / 4) * (TB / 4) / 512;   // 4x4 byte blocks per thread
+    constexpr int NUW = (SC / 4) * KP * 2 / 512;     // uint4 of U/W per thread
+    static_assert((SC / 4) * (TB / 4) % 512 == 0 && (SC / 4) * KP * 2 % 512 == 0, "tile");
+    uint32_t rpk[NPK][4];
+    u32x4 ruw[NUW];
+#define DPCA_ZT_LOAD(c0)                                                                    \
+    do {                                                                                    \
+        _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
+            const int it_ = tid + 512 * u;                                                  \
+            const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
+            const uint8_t *p_ = packed + ((c0) + 4 * sg_) * ldp + mb + 4 * bq_;             \
+            _Pragma("unroll") for (int j = 0; j < 4; j++) rpk[u][j] =                      \
+                *reinterpret_cast<const uint32_t *>(p_ + j * ldp);                          \
+        }                                                                                   \
+        const u32x4 *q_ = reinterpret_cast<const u32x4 *>(UWB) + ((c0) / 4) * KP * 2;      \
+        _Pragma("unroll") for (int u = 0; u < NUW; u++) ruw[u] = q_[tid + 512 * u];        \
+    } while (0)
+    DPCA_ZT_LOAD(sbeg < send ? sbeg : 0);
+
     for (int64_t s0 = sbeg; s0 < send; s0 += SC) {
         __syncthreads();
-        // packed: (SC/4) SNP groups x (TB/4) dword columns, one 4x4 byte
-        // transpose per item
-        for (int it = tid; it < (SC / 4) * (TB / 4); it += 512) {
+        // 4x4 byte transpose of each block into [byte column][SNP group]
+#pragma unroll
+        for (int u = 0; u < NPK; u++) {
+            const int it = tid + 512 * u;
             const int sg = it / (TB / 4), bq = it % (TB / 4);
-            const uint8_t *src = packed + (s0 + 4 * sg) * ldp + mb + 4 * bq;
-            const uint32_t r0 = *reinterpret_cast<const uint32_t *>(src);
-            const uint32_t r1 = *reinterpret_cast<const uint32_t *>(src + ldp);
-            const uint32_t r2 = *reinterpret_cast<const uint32_t *>(src + 2 * ldp);
-            const uint32_t r3 = *reinterpret_cast<const uint32_t *>(src + 3 * ldp);
-            // o_j = [r0.b_j, r1.b_j, r2.b_j, r3.b_j]
-            const uint32_t lo01 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);  // r0b0 r1b0 r0b1 r1b1
-            const uint32_t hi01 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);  // r0b2 r1b2 r0b3 r1b3
+            const uint32_t r0 = rpk[u][0], r1 = rpk[u][1], r2 = rpk[u][2], r3 = rpk[u][3];
+            const uint32_t lo01 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);
+            const uint32_t hi01 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);
             const uint32_t lo23 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
             const uint32_t hi23 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
             pk[4 * bq + 0][sg] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
@@ -305,20 +339,22 @@ __global__ __launch_bounds__(512) void k_zt_mx(const uint8_t *__restrict__ packe
             pk[4 * bq + 2][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
             pk[4 * bq + 3][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
         }
-        {
-            const uint4 *src = UWB + (s0 / 4) * KP * 2;
-            for (int e = tid; e < (SC / 4) * KP * 2; e += 512) uw[e] = src[e];
-        }
+#pragma unroll
+        for (int u = 0; u < NUW; u++) uw[tid + 512 * u] = ruw[u];
         __syncthreads();
+        {
+            // unconditional (re-reads the last chunk at the end) so the
+            // prefetch registers stay registers
+            const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
+            DPCA_ZT_LOAD(nx);
+        }
 #pragma unroll 1
         for (int t = 0; t < SC / 16; t++) {
             u32x4 bh[CT], bl[CT];
 #pragma unroll
             for (int c = 0; c < CT; c++) {
-                const uint4 h = uw[((4 * t + g) * KP + c * 16 + r) * 2];
-                const uint4 l = uw[((4 * t + g) * KP + c * 16 + r) * 2 + 1];
-                bh[c] = u32x4{h.x, h.y, h.z, h.w};
-                bl[c] = u32x4{l.x, l.y, l.z, l.w};
+                bh[c] = uw[((4 * t + g) * 2) * KP + c * 16 + r];
+                bl[c] = uw[((4 * t + g) * 2 + 1) * KP + c * 16 + r];
             }
 #pragma unroll
             for (int a = 0; a < SA; a++) {
