@@ -284,7 +284,10 @@ __global__ __launch_bounds__(512) void k_zt_mx(const uint8_t *__restrict__ packe
     constexpr int KP = CT * 16;
     constexpr int TB = SA * 16 * 8 / 4;    // packed bytes per row in this tile
     constexpr int SC = 128;                // SNPs per chunk
-    __shared__ uint32_t pk[TB][SC / 4];    // [byte column][4-SNP group]
+    // [byte column][4-SNP group], rows padded by one dword so that the
+    // transposing stores (consecutive byte columns) and the reads (4 byte
+    // columns x 4 groups per 16 lanes) spread over the LDS banks
+    __shared__ uint32_t pk[TB][SC / 4 + 1];
     __shared__ u32x4 uw[(SC / 4) * KP * 2];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
