@@ -44,6 +44,7 @@ class Report(ctypes.Structure):
         ("t_mm", ctypes.c_double), ("t_mm_a", ctypes.c_double), ("t_mm_at", ctypes.c_double),
         ("t_gs", ctypes.c_double), ("t_rr", ctypes.c_double), ("t_other", ctypes.c_double),
         ("wall", ctypes.c_double), ("last_crit", ctypes.c_double),
+        ("t_h2d", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -245,9 +246,9 @@ class PCA:
 
     def bench_pass(self, Q: np.ndarray, npass: int):
         """Event-timed passes: (ms per pass, ms Z Q kernel, ms Z^T T kernel, ms other,
-        host wall ms per pass)."""
+        host wall ms per pass, host-to-device copy ms per pass)."""
         Q = np.ascontiguousarray(Q, dtype=np.float64)
-        ms = np.zeros(5, np.float64)
+        ms = np.zeros(6, np.float64)
         _check(lib().dpca_bench_pass(self._h, _ptr(Q), npass, _ptr(ms)))
         return ms
 

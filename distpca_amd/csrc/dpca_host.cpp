@@ -137,7 +137,7 @@ struct Timer {
     int kind;
     hipEvent_t a, b;
 };
-enum { T_LOAD, T_MM_A, T_MM_AT, T_MM, T_GS, T_RR, T_OTHER, T_NKIND };
+enum { T_LOAD, T_MM_A, T_MM_AT, T_MM, T_GS, T_RR, T_OTHER, T_H2D, T_NKIND };
 
 struct dpca_ctx {
     dpca_params p{};
@@ -165,6 +165,16 @@ struct dpca_ctx {
     float *Tpart = nullptr, *Apart = nullptr;
     int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 8;
     int64_t seg_bytes = 0, rps = 0;
+    // streaming from pinned host memory (matrix above device_cap)
+    bool streamed = false;
+    uint8_t *hpacked = nullptr;       // pinned, Npad x ldp, device layout
+    uint8_t *dbuf[2] = {nullptr, nullptr};
+    int64_t chunk_rows = 0;           // rows per streamed block (multiple of 1024)
+    int64_t block_rows = 0;           // rows the per-block buffers are sized for
+    hipStream_t cst = nullptr;        // copy stream
+    hipEvent_t ev_copied[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+    bool stats_done = false;
+    double t_h2d = 0, bytes_h2d = 0, t_stream_pass = 0;
     int *flags = nullptr;             // [0] shifted used, [1] QR failure, [2] Jacobi
     IterState *its = nullptr;
     std::vector<double> crit_hist;
@@ -190,17 +200,18 @@ static hipEvent_t get_event(dpca_ctx *c)
 
 struct Stage {
     dpca_ctx *c;
+    hipStream_t s;
     Timer tm;
-    Stage(dpca_ctx *ctx, int kind) : c(ctx)
+    Stage(dpca_ctx *ctx, int kind, hipStream_t on = nullptr) : c(ctx), s(on ? on : ctx->st)
     {
         tm.kind = kind;
         tm.a = get_event(c);
         tm.b = get_event(c);
-        if (tm.a) (void)hipEventRecord(tm.a, c->st);
+        if (tm.a) (void)hipEventRecord(tm.a, s);
     }
     ~Stage()
     {
-        if (tm.b) (void)hipEventRecord(tm.b, c->st);
+        if (tm.b) (void)hipEventRecord(tm.b, s);
         c->pending.push_back(tm);
     }
 };
@@ -241,6 +252,13 @@ static void free_ctx(dpca_ctx *c)
                     c->Tpart, c->Apart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (int b = 0; b < 2; b++) {
+        if (c->dbuf[b]) (void)hipFree(c->dbuf[b]);
+        if (c->ev_copied[b]) (void)hipEventDestroy(c->ev_copied[b]);
+        if (c->ev_free[b]) (void)hipEventDestroy(c->ev_free[b]);
+    }
+    if (c->hpacked) (void)hipHostFree(c->hpacked);
+    if (c->cst) (void)hipStreamDestroy(c->cst);
     for (auto &tm : c->pending) {
         if (tm.a) (void)hipEventDestroy(tm.a);
         if (tm.b) (void)hipEventDestroy(tm.b);
@@ -313,6 +331,7 @@ static int resolve_params(dpca_ctx *c, const dpca_params *in)
 static int alloc_work(dpca_ctx *c)
 {
     const size_t mk = (size_t)c->Mpad * c->kp * sizeof(double);
+    const int64_t B = c->block_rows;  // rows of the largest block a pass works on
     int r = 0;
     r |= dalloc(&c->zval, (size_t)c->Npad * 4 * sizeof(double));
     r |= dalloc(&c->mu, (size_t)c->Npad * sizeof(double));
@@ -325,33 +344,33 @@ static int alloc_work(dpca_ctx *c)
     r |= dalloc(&c->RHS2old, mk);
     r |= dalloc(&c->tmp, mk);
     r |= dalloc(&c->T, (size_t)c->Npad * c->kp * sizeof(double));
-    // K_B splits: aim for >= ~2048 workgroups, each split >= 64 rows
+    // FP64 Z^T T: aim for >= ~2048 workgroups, each split >= 256 rows
     const int64_t tiles = c->ldp / 64;
     int64_t sp = (2048 + tiles - 1) / tiles;
-    sp = std::max<int64_t>(1, std::min<int64_t>(sp, c->Npad / 256));
+    sp = std::max<int64_t>(1, std::min<int64_t>(sp, B / 256));
     c->splits = (int)sp;
     if (c->p.precision == DPCA_FP64) {
         r |= dalloc(&c->part, (size_t)c->splits * mk);
     } else {
         // T = Z Q: sample segments of <= 16384 samples (FP32 accumulation
-        // length), 8 or 2 SNP tiles per wave depending on how many
+        // length), 4 or 2 SNP tiles per wave depending on how many
         // workgroups that leaves.
         const int64_t nsegs = (c->ldp + 4095) / 4096;
         c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 64);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
-        c->st_tiles = (c->Npad / 512 * c->nseg >= 512) ? 4 : 2;
+        c->st_tiles = (B / 512 * c->nseg >= 512) ? 4 : 2;
         if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 2, 4, 8
         // Z^T T: SNP splits of <= 16384 rows, >= ~1024 workgroups
         c->sa_tiles = (c->ldp >= 256 * 32) ? 8 : 2;
-        const int64_t tiles = c->ldp / (c->sa_tiles * 32);
-        int64_t ns = std::max<int64_t>((1024 + tiles - 1) / tiles, (c->Npad + 16383) / 16384);
-        ns = std::min<int64_t>(ns, c->Npad / 128);
-        c->rps = round_up((c->Npad + ns - 1) / ns, 128);
-        c->nsplit = (int)((c->Npad + c->rps - 1) / c->rps);
+        const int64_t tl = c->ldp / (c->sa_tiles * 32);
+        int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 16383) / 16384);
+        ns = std::min<int64_t>(ns, B / 128);
+        c->rps = round_up((B + ns - 1) / ns, 128);
+        c->nsplit = (int)((B + c->rps - 1) / c->rps);
         r |= dalloc(&c->ztab, (size_t)c->Npad * sizeof(uint4));
         r |= dalloc(&c->QB, (size_t)c->Mpad / 8 * c->kp * 2 * sizeof(uint4));
         r |= dalloc(&c->UWB, (size_t)c->Npad / 4 * c->kp * 2 * sizeof(uint4));
-        r |= dalloc(&c->Tpart, (size_t)c->nseg * c->Npad * c->kp * sizeof(float));
+        r |= dalloc(&c->Tpart, (size_t)c->nseg * B * c->kp * sizeof(float));
         r |= dalloc(&c->Apart, (size_t)c->nsplit * c->Mpad * c->kp * sizeof(float));
     }
     r |= dalloc(&c->gpart, (size_t)c->ngparts * 64 * 64 * sizeof(double));
@@ -371,13 +390,25 @@ static int alloc_work(dpca_ctx *c)
     return 0;
 }
 
-static int compute_stats(dpca_ctx *c)
+// Per-SNP statistics of rows [r0, r0+rows) held at `blk` (first touch of
+// those SNPs, as src/methods.cpp:819-836).
+static int block_stats(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows)
 {
     Stage s(c, T_LOAD);
-    DPCA_CHECK(launch_snp_stats(c->st, c->packed, c->Npad, c->ldp, 1.0 / sqrt((double)c->N),
-                                c->ssum, c->sctr, c->zval, c->mu, c->dsc));
-    if (c->p.precision == DPCA_MIXED) DPCA_CHECK(launch_ztab(c->st, c->zval, c->Npad, c->ztab));
+    DPCA_CHECK(launch_snp_stats(c->st, blk, rows, c->ldp, 1.0 / sqrt((double)c->N),
+                                c->ssum + r0, c->sctr + r0, c->zval + 4 * r0, c->mu + r0,
+                                c->dsc + r0));
+    if (c->p.precision == DPCA_MIXED)
+        DPCA_CHECK(launch_ztab(c->st, c->zval + 4 * r0, rows, c->ztab + r0));
     return 0;
+}
+
+static int compute_stats(dpca_ctx *c)
+{
+    if (c->streamed) return 0;  // computed on the first streamed pass
+    int r = block_stats(c, c->packed, 0, c->Npad);
+    if (!r) c->stats_done = true;
+    return r;
 }
 
 static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out)
@@ -412,11 +443,45 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out)
         delete c;
         return -2;
     }
+    const int64_t bytes = c->Npad * c->ldp;
+    c->streamed = (c->p.device_cap > 0 && bytes > c->p.device_cap);
+    c->block_rows = c->Npad;
+    if (c->streamed) {
+        // blocks of ~512 MB (or a quarter of the cap), whole multiples of 1024
+        // rows; two of them in HBM at a time
+        int64_t target = std::min<int64_t>(512ll << 20, c->p.device_cap / 4);
+        c->chunk_rows = std::max<int64_t>(1024, target / c->ldp / 1024 * 1024);
+        c->chunk_rows = std::min(c->chunk_rows, c->Npad);
+        c->block_rows = c->chunk_rows;
+        if (hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking) != hipSuccess) {
+            set_error("hipStreamCreate failed");
+            free_ctx(c);
+            return -2;
+        }
+        for (int b = 0; b < 2; b++) {
+            if ((r = dalloc(&c->dbuf[b], (size_t)(c->chunk_rows * c->ldp)))) {
+                free_ctx(c);
+                return r;
+            }
+            if (hipEventCreateWithFlags(&c->ev_copied[b], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_free[b], hipEventDisableTiming) != hipSuccess) {
+                set_error("hipEventCreate failed");
+                free_ctx(c);
+                return -2;
+            }
+            (void)hipEventRecord(c->ev_free[b], c->st);
+        }
+        if (hipHostMalloc((void **)&c->hpacked, (size_t)bytes) != hipSuccess) {
+            set_error("hipHostMalloc(%lld bytes) for the streamed matrix failed", (long long)bytes);
+            free_ctx(c);
+            return -3;
+        }
+    }
     if ((r = alloc_work(c))) {
         free_ctx(c);
         return r;
     }
-    if ((r = dalloc(&c->packed, (size_t)c->Npad * c->ldp))) {
+    if (!c->streamed && (r = dalloc(&c->packed, (size_t)bytes))) {
         free_ctx(c);
         return r;
     }
@@ -424,9 +489,51 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out)
     return 0;
 }
 
+// .bed rows (np bytes) -> device layout rows (ldp bytes, padding codes 01)
+// on the host, for the streamed matrix (same bytes as k_repack).
+static void host_repack_rows(const uint8_t *raw, int64_t n, int64_t np, int64_t ldp, int M,
+                             uint8_t *dst)
+{
+    const int valid = M - 4 * (int)(np - 1);
+    const uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
+    const int T = host_threads();
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++)
+        th.emplace_back([=] {
+            for (int64_t r = t; r < n; r += T) {
+                uint8_t *d = dst + r * ldp;
+                memcpy(d, raw + r * np, (size_t)np);
+                if (valid < 4) d[np - 1] = (uint8_t)((d[np - 1] & keep) | (kPadByte & ~keep));
+                memset(d + np, kPadByte, (size_t)(ldp - np));
+            }
+        });
+    for (auto &x : th) x.join();
+}
+
+// Streamed matrix: per-SNP statistics without a pass (one copy sweep).
+static int stream_stats_sweep(dpca_ctx *c)
+{
+    for (int64_t r0 = 0; r0 < c->Npad; r0 += c->chunk_rows) {
+        const int64_t rows = std::min(c->chunk_rows, c->Npad - r0);
+        DPCA_CHECK(hipMemcpyAsync(c->dbuf[0], c->hpacked + r0 * c->ldp, (size_t)(rows * c->ldp),
+                                  hipMemcpyHostToDevice, c->st));
+        int r = block_stats(c, c->dbuf[0], r0, rows);
+        if (r) return r;
+        DPCA_CHECK(hipStreamSynchronize(c->st));
+    }
+    c->stats_done = true;
+    return 0;
+}
+
 // Raw .bed rows from host memory -> padded device layout, in chunks.
 static int upload_rows(dpca_ctx *c, const uint8_t *host, int64_t rows)
 {
+    if (c->streamed) {
+        Stage s(c, T_LOAD);
+        host_repack_rows(host, rows, c->np, c->ldp, c->M, c->hpacked);
+        memset(c->hpacked + rows * c->ldp, kPadByte, (size_t)((c->Npad - rows) * c->ldp));
+        return 0;
+    }
     Stage s(c, T_LOAD);
     const int64_t chunk_rows = std::max<int64_t>(1, (256ll << 20) / c->np);
     uint8_t *raw = nullptr;
@@ -453,75 +560,114 @@ static int upload_rows(dpca_ctx *c, const uint8_t *host, int64_t rows)
 }
 
 // --------------------------------------------------------------- one pass --
-// ARHS = Z^T (Z Qin) over the resident matrix (src/methods.cpp:786-936).
-static int pass_fp64(dpca_ctx *c, const double *Qin, double *out)
+// A pass ARHS = Z^T (Z Qin) (src/methods.cpp:786-936) is a loop over blocks
+// of SNP rows: the whole resident matrix as one block, or ~512 MB blocks
+// streamed from pinned host memory.  Each block adds its Z_b^T (Z_b Qin)
+// into `out` (FP64, block order); `half` stops after T = Z_b Qin.
+
+// Mixed form: Q -> bf16 hi/lo once per pass.
+static int pass_prologue(dpca_ctx *c, const double *Qin)
 {
-    {
-        Stage s(c, T_MM_A);
-        DPCA_CHECK(launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, Qin, c->kp, c->T));
-    }
-    {
-        Stage s(c, T_MM_AT);
-        DPCA_CHECK(launch_zt_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->T, c->kp,
-                                 c->splits, c->part));
-    }
-    {
-        Stage s(c, T_MM);
-        DPCA_CHECK(launch_reduce_parts(c->st, c->part, c->splits, c->Mpad * c->kp, out, 0));
-    }
-    c->passes++;
+    if (c->p.precision != DPCA_MIXED) return 0;
+    Stage s(c, T_MM);
+    DPCA_CHECK(launch_qsplit(c->st, Qin, c->Mpad, c->kp, c->QB));
     return 0;
 }
 
-// Mixed form: T = Z Q in bf16 hi+lo (FP64 T), then Z^T T on the raw codes.
-static int half_pass_mixed(dpca_ctx *c, const double *Qin)
+static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
+                      const double *Qin, double *out, bool accumulate, bool half)
 {
-    {
+    double *T = c->T + r0 * c->kp;
+    if (c->p.precision == DPCA_FP64) {
+        {
+            Stage s(c, T_MM_A);
+            DPCA_CHECK(launch_za_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, Qin, c->kp, T));
+        }
+        if (half) return 0;
+        {
+            Stage s(c, T_MM_AT);
+            DPCA_CHECK(launch_zt_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, T, c->kp,
+                                     c->splits, c->part));
+        }
         Stage s(c, T_MM);
-        DPCA_CHECK(launch_qsplit(c->st, Qin, c->Mpad, c->kp, c->QB));
+        DPCA_CHECK(launch_reduce_parts(c->st, c->part, c->splits, c->Mpad * c->kp, out,
+                                       accumulate ? 1 : 0));
+        return 0;
     }
+    uint4 *UWB = c->UWB + (r0 / 4) * c->kp * 2;
     {
         Stage s(c, T_MM_A);
-        DPCA_CHECK(launch_za_mx(c->st, c->packed, c->Npad, c->ldp, c->ztab, c->QB, c->kp,
+        DPCA_CHECK(launch_za_mx(c->st, blk, rows, c->ldp, c->ztab + r0, c->QB, c->kp,
                                 c->st_tiles, c->seg_bytes, c->nseg, c->Tpart));
     }
     {
         Stage s(c, T_MM);
-        DPCA_CHECK(launch_tprep(c->st, c->Tpart, c->nseg, c->Npad, c->kp, c->mu, c->dsc, c->T,
-                                c->UWB));
+        DPCA_CHECK(launch_tprep(c->st, c->Tpart, c->nseg, rows, c->kp, c->mu + r0, c->dsc + r0,
+                                T, UWB));
     }
+    if (half) return 0;
+    const int nsplit = (int)((rows + c->rps - 1) / c->rps);
+    {
+        Stage s(c, T_MM_AT);
+        DPCA_CHECK(launch_zt_mx(c->st, blk, rows, c->ldp, UWB, c->kp, c->sa_tiles, c->rps,
+                                nsplit, c->Apart));
+    }
+    Stage s(c, T_MM);
+    DPCA_CHECK(launch_reduce_f32(c->st, c->Apart, nsplit, c->Mpad * c->kp, out,
+                                 accumulate ? 1 : 0));
     return 0;
 }
 
-static int pass_mixed(dpca_ctx *c, const double *Qin, double *out)
+// Streamed pass: block j is copied on the copy stream into buffer j&1 while
+// block j-1 is multiplied on the compute stream; events order the reuse of
+// each buffer.  The first streamed pass also computes the per-SNP statistics
+// of each block before its products (first touch, src/methods.cpp:819-836).
+static int pass_streamed(dpca_ctx *c, const double *Qin, double *out, bool half)
 {
-    int r = half_pass_mixed(c, Qin);
-    if (r) return r;
-    {
-        Stage s(c, T_MM_AT);
-        DPCA_CHECK(launch_zt_mx(c->st, c->packed, c->Npad, c->ldp, c->UWB, c->kp, c->sa_tiles,
-                                c->rps, c->nsplit, c->Apart));
+    const double t0 = now_s();
+    const int64_t nblk = (c->Npad + c->chunk_rows - 1) / c->chunk_rows;
+    for (int64_t j = 0; j < nblk; j++) {
+        const int b = (int)(j & 1);
+        const int64_t r0 = j * c->chunk_rows;
+        const int64_t rows = std::min(c->chunk_rows, c->Npad - r0);
+        DPCA_CHECK(hipStreamWaitEvent(c->cst, c->ev_free[b], 0));
+        {
+            Stage s(c, T_H2D, c->cst);
+            DPCA_CHECK(hipMemcpyAsync(c->dbuf[b], c->hpacked + r0 * c->ldp, (size_t)(rows * c->ldp),
+                                      hipMemcpyHostToDevice, c->cst));
+        }
+        c->bytes_h2d += (double)(rows * c->ldp);
+        DPCA_CHECK(hipEventRecord(c->ev_copied[b], c->cst));
+        DPCA_CHECK(hipStreamWaitEvent(c->st, c->ev_copied[b], 0));
+        int r;
+        if (!c->stats_done && (r = block_stats(c, c->dbuf[b], r0, rows))) return r;
+        if ((r = pass_block(c, c->dbuf[b], r0, rows, Qin, out, j > 0, half))) return r;
+        DPCA_CHECK(hipEventRecord(c->ev_free[b], c->st));
     }
-    {
-        Stage s(c, T_MM);
-        DPCA_CHECK(launch_reduce_f32(c->st, c->Apart, c->nsplit, c->Mpad * c->kp, out));
-    }
-    c->passes++;
+    c->stats_done = true;
+    c->t_stream_pass += now_s() - t0;  // host-side issue time only
     return 0;
 }
 
 static int do_pass(dpca_ctx *c, const double *Qin, double *out)
 {
-    return c->p.precision == DPCA_MIXED ? pass_mixed(c, Qin, out) : pass_fp64(c, Qin, out);
+    int r = pass_prologue(c, Qin);
+    if (r) return r;
+    if (c->streamed)
+        r = pass_streamed(c, Qin, out, false);
+    else
+        r = pass_block(c, c->packed, 0, c->Npad, Qin, out, false, false);
+    if (!r) c->passes++;
+    return r;
 }
 
 // Z Q only (the in-core variant's half pass), into c->T.
 static int half_pass(dpca_ctx *c, const double *Qin)
 {
-    if (c->p.precision == DPCA_MIXED) return half_pass_mixed(c, Qin);
-    Stage s(c, T_MM_A);
-    DPCA_CHECK(launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, Qin, c->kp, c->T));
-    return 0;
+    int r = pass_prologue(c, Qin);
+    if (r) return r;
+    if (c->streamed) return pass_streamed(c, Qin, nullptr, true);
+    return pass_block(c, c->packed, 0, c->Npad, Qin, nullptr, false, true);
 }
 
 // CholQR2 (+ shifted fallback) of A (Mpad x kp) into c->Q; uses c->tmp.
@@ -567,6 +713,7 @@ static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
     const dpca_params &p = c->p;
     for (int i = 0; i < T_NKIND; i++) c->t[i] = 0;
     c->passes = 0;
+    c->bytes_h2d = 0;
     c->crit_hist.clear();
     std::vector<double> om;
     double *rhs_in = rhs;
@@ -683,6 +830,9 @@ static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
         rep->t_other = c->t[T_OTHER];
         rep->wall = now_s() - t0;
         rep->last_crit = hs.crit;
+        rep->streamed = c->streamed;
+        rep->t_h2d = c->t[T_H2D];
+        rep->h2d_bytes = c->bytes_h2d;
     }
     return 0;
 }
@@ -804,6 +954,29 @@ int dpca_open(const char *bfile, const dpca_params *p, dpca_ctx **out)
     }
     c->iids.reserve(M);
     for (auto &l : fam) c->iids.push_back(token(l, 1));
+    fseek(fb, 3, SEEK_SET);  // 3-byte magic, not validated (src/distpca.cpp:105)
+    if (c->streamed) {
+        // the matrix stays in pinned host memory in the device row layout
+        const int64_t chunk = std::max<int64_t>(1, (256ll << 20) / np);
+        std::vector<uint8_t> buf((size_t)(std::min(chunk, N) * np));
+        for (int64_t r0 = 0; r0 < N && !r; r0 += chunk) {
+            const int64_t n = std::min(chunk, N - r0);
+            if (fread(buf.data(), 1, (size_t)(n * np), fb) != (size_t)(n * np)) {
+                set_error("short read on %s", bed.c_str());
+                r = -1;
+                break;
+            }
+            host_repack_rows(buf.data(), n, np, c->ldp, M, c->hpacked + r0 * c->ldp);
+        }
+        fclose(fb);
+        if (!r) memset(c->hpacked + N * c->ldp, kPadByte, (size_t)((c->Npad - N) * c->ldp));
+        if (r) {
+            free_ctx(c);
+            return r;
+        }
+        *out = c;
+        return 0;
+    }
     // read in pinned chunks and stage
     const int64_t chunk_rows = std::max<int64_t>(1, (256ll << 20) / np);
     uint8_t *pin = nullptr;
@@ -820,7 +993,6 @@ int dpca_open(const char *bfile, const dpca_params *p, dpca_ctx **out)
         free_ctx(c);
         return -3;
     }
-    fseek(fb, 3, SEEK_SET);  // 3-byte magic, not validated (src/distpca.cpp:105)
     {
         Stage s(c, T_LOAD);
         for (int64_t r0 = 0; r0 < N && !r; r0 += chunk_rows) {
@@ -915,8 +1087,19 @@ int dpca_open_synthetic(const dpca_gen_spec *g, const dpca_params *p, dpca_ctx *
     if (e == hipSuccess && nw) e = hipMemcpy(dwts, tab.wts, nw * 2, hipMemcpyHostToDevice);
     {
         Stage s(c, T_LOAD);
-        if (e == hipSuccess)
+        if (c->streamed) {
+            // generate block by block in HBM and park it in pinned host memory
+            for (int64_t r0 = 0; r0 < c->Npad && e == hipSuccess; r0 += c->chunk_rows) {
+                const int64_t rows = std::min(c->chunk_rows, c->Npad - r0);
+                e = launch_generate(c->st, spec, dthr, dwts, r0, rows, c->dbuf[0], c->ldp);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(c->hpacked + r0 * c->ldp, c->dbuf[0], (size_t)(rows * c->ldp),
+                                       hipMemcpyDeviceToHost, c->st);
+                if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+            }
+        } else if (e == hipSuccess) {
             e = launch_generate(c->st, spec, dthr, dwts, 0, c->Npad, c->packed, c->ldp);
+        }
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     (void)hipFree(dthr);
@@ -1001,6 +1184,10 @@ int dpca_snp_stats(dpca_ctx *c, long long *sum, long long *ctr, double *zval)
 {
     if (!c) return -1;
     (void)hipSetDevice(c->dev);
+    if (!c->stats_done) {
+        int r = stream_stats_sweep(c);
+        if (r) return r;
+    }
     if (sum) DPCA_CHECK(hipMemcpy(sum, c->ssum, (size_t)c->N * 8, hipMemcpyDeviceToHost));
     if (ctr) DPCA_CHECK(hipMemcpy(ctr, c->sctr, (size_t)c->N * 8, hipMemcpyDeviceToHost));
     if (zval) DPCA_CHECK(hipMemcpy(zval, c->zval, (size_t)c->N * 32, hipMemcpyDeviceToHost));
@@ -1014,13 +1201,25 @@ int dpca_decode(dpca_ctx *c, long long snp0, long long rows, uint8_t *out)
         return -1;
     }
     (void)hipSetDevice(c->dev);
-    uint8_t *d = nullptr;
+    uint8_t *d = nullptr, *src = nullptr;
     if (dalloc(&d, (size_t)rows * c->M)) return -3;
-    hipError_t e = launch_decode_codes(c->st, c->packed + snp0 * c->ldp, rows, c->ldp, c->M, d);
+    hipError_t e = hipSuccess;
+    const uint8_t *rowsrc = c->packed ? c->packed + snp0 * c->ldp : nullptr;
+    if (c->streamed) {
+        if (dalloc(&src, (size_t)(rows * c->ldp))) {
+            (void)hipFree(d);
+            return -3;
+        }
+        e = hipMemcpyAsync(src, c->hpacked + snp0 * c->ldp, (size_t)(rows * c->ldp),
+                           hipMemcpyHostToDevice, c->st);
+        rowsrc = src;
+    }
+    if (e == hipSuccess) e = launch_decode_codes(c->st, rowsrc, rows, c->ldp, c->M, d);
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, d, (size_t)rows * c->M, hipMemcpyDeviceToHost, c->st);
     if (e == hipSuccess) e = hipStreamSynchronize(c->st);
     (void)hipFree(d);
+    if (src) (void)hipFree(src);
     if (e != hipSuccess) {
         set_error("dpca_decode: %s", hipGetErrorString(e));
         return -2;
@@ -1035,8 +1234,13 @@ int dpca_copy_packed(dpca_ctx *c, long long snp0, long long rows, uint8_t *out)
         return -1;
     }
     (void)hipSetDevice(c->dev);
-    DPCA_CHECK(hipMemcpy2D(out, c->np, c->packed + snp0 * c->ldp, c->ldp, c->np, rows,
-                           hipMemcpyDeviceToHost));
+    if (c->streamed) {
+        for (long long r = 0; r < rows; r++)
+            memcpy(out + r * c->np, c->hpacked + (snp0 + r) * c->ldp, (size_t)c->np);
+    } else {
+        DPCA_CHECK(hipMemcpy2D(out, c->np, c->packed + snp0 * c->ldp, c->ldp, c->np, rows,
+                               hipMemcpyDeviceToHost));
+    }
     // the device copy holds missing codes past sample M; the file holds 00
     const int valid = c->M - 4 * (int)(c->np - 1);
     if (valid < 4) {
@@ -1228,12 +1432,13 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
     c->free_ev.push_back(e0);
     c->free_ev.push_back(e1);
     // per-kernel times: the stage events bracket the kernels of each stage
-    double ka = 0, kb = 0;
+    double ka = 0, kb = 0, kc = 0;
     for (auto &tm : c->pending) {
         float t = 0;
         if (tm.a && tm.b && hipEventElapsedTime(&t, tm.a, tm.b) == hipSuccess) {
             if (tm.kind == T_MM_A) ka += t;
             if (tm.kind == T_MM_AT) kb += t;
+            if (tm.kind == T_H2D) kc += t;
         }
     }
     collect_timers(c);
@@ -1242,6 +1447,7 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
     ms[2] = kb / npass;
     ms[3] = ms[0] - ms[1] - ms[2];
     ms[4] = hwall * 1e3 / npass;
+    ms[5] = kc / npass;
     return 0;
 }
 

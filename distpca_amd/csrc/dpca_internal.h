@@ -79,7 +79,7 @@ hipError_t launch_zt_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int
                         const uint4 *UWB, int kp, int sa_tiles, int64_t rows_per_split,
                         int nsplit, float *part);
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
-                             double *out);
+                             double *out, int accumulate);
 
 // Tall-skinny helpers, FP64.  X, Y: m x kp row-major.
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,

@@ -417,21 +417,21 @@ hipError_t launch_zt_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int
     return hipErrorInvalidValue;
 }
 
-// out[e] = sum over splits of float partials, in FP64, split order.
+// out[e] (+)= sum over splits of float partials, in FP64, split order.
 __global__ void k_reduce_f32(const float *__restrict__ part, int splits, int64_t n,
-                             double *__restrict__ out)
+                             double *__restrict__ out, int accumulate)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= n) return;
-    double s = 0;
+    double s = accumulate ? out[e] : 0.0;
     for (int p = 0; p < splits; p++) s += (double)part[(int64_t)p * n + e];
     out[e] = s;
 }
 
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
-                             double *out)
+                             double *out, int accumulate)
 {
-    k_reduce_f32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, n, out);
+    k_reduce_f32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, n, out, accumulate);
     return hipGetLastError();
 }
 

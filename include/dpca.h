@@ -61,6 +61,8 @@ typedef struct dpca_report {
     double t_generate_rhs, t_load, t_mm, t_mm_a, t_mm_at, t_gs, t_rr, t_other;
     double wall;
     double last_crit;       /* last criterion value (crit 0/2) or count (1)  */
+    double t_h2d;           /* streamed: seconds of host-to-device copies    */
+    double h2d_bytes;       /* streamed: bytes copied host-to-device         */
 } dpca_report;
 
 /* Defaults of initialize_structure (src/utilities.cpp:195-232). */
@@ -153,7 +155,8 @@ int dpca_syev(const double *B, int k, double *w, double *X, int device);
  * library's stream, in milliseconds: ms[0] whole pass, ms[1] the Z Q kernel,
  * ms[2] the Z^T T kernel, ms[3] everything else in the pass (operand
  * splits, partial reductions), ms[4] host wall clock per pass (launch to
- * stream drain).  ms must hold 5 values. */
+ * stream drain), ms[5] host-to-device copy time per pass (streamed
+ * matrices; it overlaps the kernels).  ms must hold 6 values. */
 int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms);
 
 void dpca_close(dpca_ctx *c);

@@ -224,3 +224,35 @@ def test_cli_dropin(tmp_path):
         a = O.principal_angles(U[:, :j], r.rhs[:, :j]).max()
         sp = O.principal_angles(ra.rhs[:, :j], r.rhs[:, :j]).max()
         assert a <= max(1e-8, 20 * sp), (j, a, sp)
+
+
+@pytest.mark.parametrize("precision", [D.FP64, D.MIXED])
+def test_streamed_matches_resident(precision):
+    # device_cap of 1 MB forces 1024-row blocks streamed from pinned host
+    # memory through two HBM buffers (the config-4 path at small scale)
+    _banner(f"streamed vs resident, precision {precision}")
+    M, N = 1000, 20000
+    spec = D.gen_spec(M, N, npop=3, fst=0.05, missing=0.01, seed=9)
+    base = dict(nsv=4, nrhs=8, power=2, crit=1, tol=1e-10, miter=8, bsize=2000,
+                precision=precision)
+    with D.PCA.synthetic(spec, D.Params(**base)) as res, \
+            D.PCA.synthetic(spec, D.Params(device_cap=1 << 20, **base)) as stm:
+        assert np.array_equal(res.packed(), stm.packed())
+        assert np.array_equal(res.decode(777, 5), stm.decode(777, 5))
+        s1, c1, z1 = res.snp_stats()
+        s2, c2, z2 = stm.snp_stats()
+        assert np.array_equal(z1.view(np.int64), z2.view(np.int64))
+        Q = np.linalg.qr(np.random.default_rng(1).standard_normal((M, 8)))[0]
+        a, b = res.pass_(Q), stm.pass_(Q)
+        tol = 1e-12 if precision == D.FP64 else 2e-6
+        assert np.abs(a - b).max() / np.abs(a).max() < tol
+        if precision == D.FP64:
+            ref = O.OOCPass(res.packed().reshape(-1), M, N, 2000)(Q)
+            assert np.abs(b - ref).max() / np.abs(ref).max() < 1e-12
+        sa, Ua, ra = res.run()
+        sb, Ub, rb = stm.run()
+    assert rb["streamed"] == 1 and ra["streamed"] == 0
+    assert rb["h2d_bytes"] >= rb["passes"] * N * ((M + 3) // 4)
+    assert ra["total_its"] == rb["total_its"]
+    assert np.allclose(sa, sb, rtol=1e-10 if precision == D.FP64 else 1e-6)
+    assert O.principal_angles(Ua[:, :2], Ub[:, :2]).max() < (1e-8 if precision == D.FP64 else 1e-4)
