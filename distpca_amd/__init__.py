@@ -246,9 +246,9 @@ class PCA:
 
     def bench_pass(self, Q: np.ndarray, npass: int):
         """Event-timed passes: (ms per pass, ms Z Q kernel, ms Z^T T kernel, ms other,
-        host wall ms per pass, host-to-device copy ms per pass)."""
+        host wall ms per pass, host-to-device copy ms per pass, small-kernel ms)."""
         Q = np.ascontiguousarray(Q, dtype=np.float64)
-        ms = np.zeros(6, np.float64)
+        ms = np.zeros(7, np.float64)
         _check(lib().dpca_bench_pass(self._h, _ptr(Q), npass, _ptr(ms)))
         return ms
 

@@ -1432,13 +1432,14 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
     c->free_ev.push_back(e0);
     c->free_ev.push_back(e1);
     // per-kernel times: the stage events bracket the kernels of each stage
-    double ka = 0, kb = 0, kc = 0;
+    double ka = 0, kb = 0, kc = 0, kd = 0;
     for (auto &tm : c->pending) {
         float t = 0;
         if (tm.a && tm.b && hipEventElapsedTime(&t, tm.a, tm.b) == hipSuccess) {
             if (tm.kind == T_MM_A) ka += t;
             if (tm.kind == T_MM_AT) kb += t;
             if (tm.kind == T_H2D) kc += t;
+            if (tm.kind == T_MM) kd += t;
         }
     }
     collect_timers(c);
@@ -1448,6 +1449,7 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
     ms[3] = ms[0] - ms[1] - ms[2];
     ms[4] = hwall * 1e3 / npass;
     ms[5] = kc / npass;
+    ms[6] = kd / npass;
     return 0;
 }
 

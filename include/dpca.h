@@ -156,7 +156,8 @@ int dpca_syev(const double *B, int k, double *w, double *X, int device);
  * ms[2] the Z^T T kernel, ms[3] everything else in the pass (operand
  * splits, partial reductions), ms[4] host wall clock per pass (launch to
  * stream drain), ms[5] host-to-device copy time per pass (streamed
- * matrices; it overlaps the kernels).  ms must hold 6 values. */
+ * matrices; it overlaps the kernels), ms[6] kernel time of the small pass
+ * kernels.  ms must hold 7 values. */
 int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms);
 
 void dpca_close(dpca_ctx *c);
