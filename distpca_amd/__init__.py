@@ -60,11 +60,37 @@ class GenSpec(ctypes.Structure):
     ]
 
 
+class ServerParams(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_longlong),
+                ("batch_capacity", ctypes.c_int), ("workers", ctypes.c_int)]
+
+
+class TicketInfo(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("status", ctypes.c_int), ("n_out", ctypes.c_longlong),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("batch", ctypes.c_longlong),
+                ("batch_samples", ctypes.c_int), ("footprint", ctypes.c_longlong)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_longlong) for f in ("batches", "batch_samples", "projections_done",
+                                                  "jobs_done", "jobs_rejected", "budget_free")]
+
+
+TICKET_PROJECT, TICKET_JOB = 1, 2
+
 EXPORTS = [
     "dpca_params_init", "dpca_open", "dpca_open_packed", "dpca_open_synthetic",
     "dpca_write_synthetic", "dpca_get_dims", "dpca_omega", "dpca_snp_stats", "dpca_decode",
     "dpca_copy_packed", "dpca_pass", "dpca_run", "dpca_crit_history", "dpca_write_outputs",
     "dpca_orthonormalize", "dpca_syev", "dpca_bench_pass", "dpca_close", "dpca_last_error",
+    "dpca_footprint", "dpca_model_from_run", "dpca_model_info", "dpca_model_project",
+    "dpca_model_free", "dpca_server_params_init", "dpca_server_create",
+    "dpca_server_submit_project", "dpca_server_submit_job", "dpca_server_wait",
+    "dpca_server_stats", "dpca_server_destroy",
 ]
 
 _lib = None
@@ -100,6 +126,20 @@ def lib():
     L.dpca_bench_pass.argtypes = [vp, vp, i32, vp]
     L.dpca_close.argtypes = [vp]
     L.dpca_close.restype = None
+    L.dpca_footprint.argtypes = [i32, i64, P(_Params), P(i64)]
+    L.dpca_model_from_run.argtypes = [vp, vp, vp, i32, P(vp)]
+    L.dpca_model_info.argtypes = [vp, P(i64), P(i32), P(i64)]
+    L.dpca_model_project.argtypes = [vp, vp, i32, vp]
+    L.dpca_model_free.argtypes = [vp]
+    L.dpca_model_free.restype = None
+    L.dpca_server_params_init.argtypes = [P(ServerParams)]
+    L.dpca_server_create.argtypes = [P(ServerParams), vp, P(vp)]
+    L.dpca_server_submit_project.argtypes = [vp, vp, i32, P(i64)]
+    L.dpca_server_submit_job.argtypes = [vp, P(GenSpec), P(_Params), P(i64)]
+    L.dpca_server_wait.argtypes = [vp, i64, vp, i64, P(TicketInfo)]
+    L.dpca_server_stats.argtypes = [vp, P(ServerCounters)]
+    L.dpca_server_destroy.argtypes = [vp]
+    L.dpca_server_destroy.restype = None
     _lib = L
     return L
 
@@ -279,3 +319,101 @@ def syev(B: np.ndarray, device: int = 0):
     X = np.empty((k, k), np.float64)
     _check(lib().dpca_syev(_ptr(B), k, _ptr(w), _ptr(X), device))
     return w, X
+
+
+def footprint(M: int, N: int, params: Params) -> int:
+    b = ctypes.c_longlong()
+    p = params.c()
+    _check(lib().dpca_footprint(M, N, ctypes.byref(p), ctypes.byref(b)))
+    return b.value
+
+
+class Model:
+    """Resident SNP loadings + training statistics (dpca_model)."""
+
+    def __init__(self, pca: PCA, rhs: np.ndarray, sigma: np.ndarray, nsv: int):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().dpca_model_from_run(pca._h, _ptr(rhs), _ptr(sigma), nsv, ctypes.byref(h)))
+        self._h = h
+        N, k, b = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_longlong()
+        lib().dpca_model_info(h, ctypes.byref(N), ctypes.byref(k), ctypes.byref(b))
+        self.N, self.nsv, self.bytes = N.value, k.value, b.value
+
+    def project(self, packed: np.ndarray, n: int) -> np.ndarray:
+        packed = np.ascontiguousarray(packed, dtype=np.uint8)
+        assert packed.size >= self.N * ((n + 3) // 4)
+        out = np.empty((n, self.nsv), np.float64)
+        _check(lib().dpca_model_project(self._h, _ptr(packed), n, _ptr(out)))
+        return out
+
+    def close(self):
+        if self._h:
+            lib().dpca_model_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """dpca_server: continuous-batching projections + HBM-admitted PCA jobs."""
+
+    def __init__(self, model: "Model | None", hbm_budget: int = 64 << 30,
+                 batch_capacity: int = 8192, workers: int = 2, device: int = 0):
+        p = ServerParams()
+        lib().dpca_server_params_init(ctypes.byref(p))
+        p.device, p.hbm_budget, p.batch_capacity, p.workers = device, hbm_budget, batch_capacity, workers
+        h = ctypes.c_void_p()
+        _check(lib().dpca_server_create(ctypes.byref(p), model._h if model else None,
+                                        ctypes.byref(h)))
+        self._h = h
+        self.model = model
+        self._keep = {}
+
+    def submit_project(self, packed: np.ndarray, n: int) -> int:
+        packed = np.ascontiguousarray(packed, dtype=np.uint8)
+        t = ctypes.c_longlong()
+        _check(lib().dpca_server_submit_project(self._h, _ptr(packed), n, ctypes.byref(t)))
+        self._keep[t.value] = (packed, n)
+        return t.value
+
+    def submit_job(self, spec: GenSpec, params: Params) -> int:
+        t = ctypes.c_longlong()
+        p = params.c()
+        _check(lib().dpca_server_submit_job(self._h, ctypes.byref(spec), ctypes.byref(p),
+                                            ctypes.byref(t)))
+        self._keep[t.value] = (None, params.nsv)
+        return t.value
+
+    def wait(self, ticket: int):
+        payload, n = self._keep.pop(ticket)
+        cap = n * self.model.nsv if payload is not None else n
+        out = np.empty(max(cap, 1), np.float64)
+        info = TicketInfo()
+        rc = lib().dpca_server_wait(self._h, ticket, _ptr(out), cap, ctypes.byref(info))
+        _check(rc)
+        res = out[:info.n_out]
+        if payload is not None:
+            res = res.reshape(n, self.model.nsv)
+        return res, info.as_dict()
+
+    def counters(self):
+        c = ServerCounters()
+        lib().dpca_server_stats(self._h, ctypes.byref(c))
+        return {f: getattr(c, f) for f, _ in c._fields_}
+
+    def close(self):
+        if self._h:
+            lib().dpca_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

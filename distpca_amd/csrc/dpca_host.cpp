@@ -26,6 +26,7 @@
 #include "dpca.h"
 #include "dpca_gen.h"
 #include "dpca_internal.h"
+#include "dpca_model.h"
 
 namespace dpca {
 
@@ -229,11 +230,18 @@ static void collect_timers(dpca_ctx *c)
     c->pending.clear();
 }
 
+// When set, dalloc only adds up sizes (footprint estimate of a context).
+static thread_local size_t *g_dry = nullptr;
+
 template <class P>
 static int dalloc(P **p, size_t bytes)
 {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
+    if (g_dry) {
+        *g_dry += bytes;
+        return 0;
+    }
     hipError_t e = hipMalloc((void **)p, bytes);
     if (e != hipSuccess) {
         set_error("hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
@@ -263,7 +271,7 @@ static void free_ctx(dpca_ctx *c)
         if (tm.a) (void)hipEventDestroy(tm.a);
         if (tm.b) (void)hipEventDestroy(tm.b);
     }
-    for (auto e : c->free_ev) hipEventDestroy(e);
+    for (auto e : c->free_ev) (void)hipEventDestroy(e);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -382,6 +390,7 @@ static int alloc_work(dpca_ctx *c)
     r |= dalloc(&c->flags, 16 * sizeof(int));
     r |= dalloc(&c->its, sizeof(IterState));
     if (r) return -3;
+    if (g_dry) return 0;
     DPCA_CHECK(hipMemsetAsync(c->Q, 0, mk, c->st));
     DPCA_CHECK(hipMemsetAsync(c->ARHS, 0, mk, c->st));
     DPCA_CHECK(hipMemsetAsync(c->RHS2, 0, mk, c->st));
@@ -1355,7 +1364,7 @@ int dpca_orthonormalize(const double *A, long long m, int k, double *Q, int *shi
         if (p) (void)hipFree(p);
     c.tmp = c.Q = c.ARHS = c.gpart = c.G = c.Rinv = nullptr;
     c.flags = nullptr;
-    for (auto e : c.free_ev) hipEventDestroy(e);
+    for (auto e : c.free_ev) (void)hipEventDestroy(e);
     c.free_ev.clear();
     (void)hipStreamDestroy(c.st);
     c.st = nullptr;
@@ -1454,5 +1463,108 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
 }
 
 void dpca_close(dpca_ctx *c) { free_ctx(c); }
+
+int dpca_footprint(int M, long long N, const dpca_params *p, long long *bytes)
+{
+    if (!p || !bytes) {
+        set_error("dpca_footprint: NULL argument");
+        return -1;
+    }
+    dpca_ctx c;
+    c.M = M;
+    c.N = N;
+    int r = resolve_params(&c, p);
+    if (r) return r;
+    size_t total = 0;
+    const int64_t packed = c.Npad * c.ldp;
+    c.streamed = (c.p.device_cap > 0 && packed > c.p.device_cap);
+    c.block_rows = c.Npad;
+    if (c.streamed) {
+        int64_t target = std::min<int64_t>(512ll << 20, c.p.device_cap / 4);
+        c.chunk_rows = std::min(c.Npad, std::max<int64_t>(1024, target / c.ldp / 1024 * 1024));
+        c.block_rows = c.chunk_rows;
+        total += 2 * (size_t)(c.chunk_rows * c.ldp);
+    } else {
+        total += (size_t)packed;
+    }
+    g_dry = &total;
+    r = alloc_work(&c);
+    g_dry = nullptr;
+    *bytes = (long long)total;
+    return r;
+}
+
+int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int nsv,
+                        dpca_model **out)
+{
+    if (!c || !rhs || !sigma || !out || nsv < 1 || nsv > c->k) {
+        set_error("dpca_model_from_run: bad arguments");
+        return -1;
+    }
+    DPCA_CHECK(hipSetDevice(c->dev));
+    for (int j = 0; j < nsv; j++)
+        if (!(sigma[j] > 0)) {
+            set_error("dpca_model_from_run: sigma[%d] is not positive", j);
+            return -1;
+        }
+    dpca_model *m = new dpca_model();
+    m->dev = c->dev;
+    m->precision = c->p.precision;
+    m->N = c->N;
+    m->Npad = c->Npad;
+    m->nsv = nsv;
+    m->kp = c->kp;
+    int r = 0;
+    const size_t n1 = (size_t)c->Npad * sizeof(double);
+    r |= dalloc(&m->zval, 4 * n1);
+    r |= dalloc(&m->mu, n1);
+    r |= dalloc(&m->dsc, n1);
+    r |= dalloc(&m->W, n1 * c->kp);
+    if (m->precision == DPCA_MIXED) r |= dalloc(&m->UWB, (size_t)c->Npad / 4 * c->kp * 2 * sizeof(uint4));
+    m->bytes = 6 * n1 + n1 * c->kp + (m->UWB ? (size_t)c->Npad / 4 * c->kp * 32 : 0);
+    double *inv = nullptr;
+    r |= dalloc(&inv, 64 * sizeof(double));
+    if (r) {
+        dpca_model_free(m);
+        return -3;
+    }
+    // U = the first nsv Ritz vectors, padded; T = Z U in FP64 over the matrix
+    std::vector<double> u((size_t)c->M * c->k, 0.0), is2(64, 0.0);
+    for (int i = 0; i < c->M; i++)
+        for (int j = 0; j < nsv; j++) u[(size_t)i * c->k + j] = rhs[(size_t)i * c->k + j];
+    for (int j = 0; j < nsv; j++) is2[j] = 1.0 / (sigma[j] * sigma[j]);
+    if (!c->stats_done && (r = stream_stats_sweep(c))) {
+        dpca_model_free(m);
+        (void)hipFree(inv);
+        return r;
+    }
+    hipError_t e = hipSuccess;
+    if ((r = copy_in_rhs(c, u.data(), c->RHS2)) == 0) {
+        if (c->streamed) {
+            const int prec = c->p.precision;
+            c->p.precision = DPCA_FP64;  // loadings in FP64 whatever the form
+            r = half_pass(c, c->RHS2);
+            c->p.precision = prec;
+        } else {
+            e = launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->RHS2, c->kp, c->T);
+        }
+    }
+    if (!r && e == hipSuccess) e = hipMemcpyAsync(inv, is2.data(), 64 * sizeof(double), hipMemcpyHostToDevice, c->st);
+    if (!r && e == hipSuccess) e = launch_loadings(c->st, c->T, c->Npad, c->kp, nsv, inv, m->W);
+    if (!r && e == hipSuccess) e = hipMemcpyAsync(m->zval, c->zval, 4 * n1, hipMemcpyDeviceToDevice, c->st);
+    if (!r && e == hipSuccess) e = hipMemcpyAsync(m->mu, c->mu, n1, hipMemcpyDeviceToDevice, c->st);
+    if (!r && e == hipSuccess) e = hipMemcpyAsync(m->dsc, c->dsc, n1, hipMemcpyDeviceToDevice, c->st);
+    if (!r && e == hipSuccess && m->UWB) e = launch_uwb(c->st, m->W, c->Npad, c->kp, m->mu, m->dsc, m->UWB);
+    if (!r && e == hipSuccess) e = hipStreamSynchronize(c->st);
+    collect_timers(c);
+    (void)hipFree(inv);
+    if (r || e != hipSuccess) {
+        if (e != hipSuccess) set_error("dpca_model_from_run: %s", hipGetErrorString(e));
+        dpca_model_free(m);
+        return r ? r : -2;
+    }
+    *out = m;
+    return 0;
+}
 
 }  // extern "C"

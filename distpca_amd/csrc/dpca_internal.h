@@ -95,6 +95,16 @@ hipError_t launch_col_dots(hipStream_t st, const double *X, const double *Y, int
 hipError_t launch_colsum(hipStream_t st, const double *X, int64_t m, int kp, double *part,
                          int nparts, double *out);
 
+// Projection batches and loadings.
+hipError_t launch_scatter_cols(hipStream_t st, const uint8_t *src, int64_t rows, int64_t nb,
+                               int n_valid, uint8_t *dst, int64_t ldb, int64_t off);
+hipError_t launch_fill_cols(hipStream_t st, uint8_t *dst, int64_t rows, int64_t ldb, int64_t c0,
+                            int64_t c1);
+hipError_t launch_loadings(hipStream_t st, const double *T, int64_t rows, int kp, int nsv,
+                           const double *inv_s2, double *W);
+hipError_t launch_uwb(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
+                      const double *dsc, uint4 *UWB);
+
 // Per-iteration state of the convergence test (device resident).
 struct IterState {
     double sigma[64];

@@ -104,6 +104,74 @@ hipError_t launch_compact_rhs(hipStream_t st, const double *src, int M, int k, i
     return hipGetLastError();
 }
 
+// Projection batches: copy a request's SNP-major payload (rows of nb bytes)
+// into byte columns [off, off+nb) of the batch matrix (row pitch ldb);
+// sample codes past n_valid in the last byte become 01 (missing).
+__global__ void k_scatter_cols(const uint8_t *__restrict__ src, int64_t rows, int64_t nb,
+                               int n_valid, uint8_t *__restrict__ dst, int64_t ldb, int64_t off)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows * nb) return;
+    const int64_t r = idx / nb, b = idx - r * nb;
+    uint8_t v = src[idx];
+    if (b == nb - 1) {
+        const int valid = n_valid - 4 * (int)(nb - 1);
+        if (valid < 4) {
+            const uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
+            v = (uint8_t)((v & keep) | (kPadByte & ~keep));
+        }
+    }
+    dst[r * ldb + off + b] = v;
+}
+
+// Fill byte columns [c0, c1) of every row with 0x55 (missing codes).
+__global__ void k_fill_cols(uint8_t *__restrict__ dst, int64_t rows, int64_t ldb, int64_t c0,
+                            int64_t c1)
+{
+    const int64_t w = c1 - c0;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows * w) return;
+    const int64_t r = idx / w;
+    dst[r * ldb + c0 + (idx - r * w)] = kPadByte;
+}
+
+hipError_t launch_scatter_cols(hipStream_t st, const uint8_t *src, int64_t rows, int64_t nb,
+                               int n_valid, uint8_t *dst, int64_t ldb, int64_t off)
+{
+    const int64_t n = rows * nb;
+    if (n == 0) return hipSuccess;
+    k_scatter_cols<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(src, rows, nb, n_valid, dst, ldb,
+                                                                off);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_cols(hipStream_t st, uint8_t *dst, int64_t rows, int64_t ldb, int64_t c0,
+                            int64_t c1)
+{
+    const int64_t n = rows * (c1 - c0);
+    if (n <= 0) return hipSuccess;
+    k_fill_cols<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(dst, rows, ldb, c0, c1);
+    return hipGetLastError();
+}
+
+// Loadings W = T diag(1/sigma^2) for the first nsv columns, 0 elsewhere.
+__global__ void k_loadings(const double *__restrict__ T, int64_t rows, int kp, int nsv,
+                           const double *__restrict__ inv_s2, double *__restrict__ W)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * kp) return;
+    const int j = (int)(e % kp);
+    W[e] = (j < nsv) ? T[e] * inv_s2[j] : 0.0;
+}
+
+hipError_t launch_loadings(hipStream_t st, const double *T, int64_t rows, int kp, int nsv,
+                           const double *inv_s2, double *W)
+{
+    const int64_t n = rows * kp;
+    k_loadings<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, nsv, inv_s2, W);
+    return hipGetLastError();
+}
+
 // --------------------------------------------------------------- SNP stats --
 // One wavefront per SNP row.  Codes are counted 16 at a time with popcounts:
 // with lo = bit0 and hi = bit1 of every 2-bit field, missing (01) = lo & ~hi,

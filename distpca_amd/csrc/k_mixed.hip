@@ -268,6 +268,38 @@ hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t ro
     return hipGetLastError();
 }
 
+// U/W operand of a resident FP64 right-hand matrix (projection loadings):
+// same layout as k_tprep's output.
+__global__ void k_uwb(const double *__restrict__ T, int64_t rows, int kp,
+                      const double *__restrict__ mu, const double *__restrict__ dsc,
+                      uint4 *__restrict__ UWB)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows / 4 * kp) return;
+    const int64_t s4 = idx / kp;
+    const int c = (int)(idx % kp);
+    uint32_t uh[4], ul[4], wh[4], wl[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int64_t s = s4 * 4 + e;
+        const double t = T[s * kp + c];
+        split_bf16(dsc[s] * t, uh[e], ul[e]);
+        split_bf16(-(dsc[s] * mu[s]) * t, wh[e], wl[e]);
+    }
+    UWB[(s4 * 2) * kp + c] = make_uint4(uh[0] | (uh[1] << 16), uh[2] | (uh[3] << 16),
+                                        wh[0] | (wh[1] << 16), wh[2] | (wh[3] << 16));
+    UWB[(s4 * 2 + 1) * kp + c] = make_uint4(ul[0] | (ul[1] << 16), ul[2] | (ul[3] << 16),
+                                            wl[0] | (wl[1] << 16), wl[2] | (wl[3] << 16));
+}
+
+hipError_t launch_uwb(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
+                      const double *dsc, uint4 *UWB)
+{
+    const int64_t n = rows / 4 * kp;
+    k_uwb<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, mu, dsc, UWB);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------ Z^T T part ----
 // Workgroup: 8 waves x SA tiles x 16 samples (SA = 8: 1024 samples = 256
 // packed bytes per row) and one SNP split (blockIdx.y).  128-SNP chunks of

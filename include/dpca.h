@@ -163,6 +163,71 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms);
 void dpca_close(dpca_ctx *c);
 const char *dpca_last_error(void);
 
+/* Device bytes a context with these sizes and parameters would allocate
+ * (no reference counterpart; used for HBM admission). */
+int dpca_footprint(int M, long long N, const dpca_params *p, long long *bytes);
+
+/* ---- Projection and serving (no reference counterpart: DistPCA stops at
+ * the left singular vectors, src/methods.cpp:1072-1118) ------------------ */
+
+typedef struct dpca_model dpca_model;
+
+/* SNP loadings W = Z U diag(1/sigma^2) (N x nsv, FP64) from a finished run
+ * on context c (rhs = the run's M x nrhs Ritz vectors, sigma its nsv singular
+ * values), with the training per-SNP table.  The model stays resident in HBM
+ * and computes in the context's form (FP64 or mixed).  Projecting a training
+ * sample returns its row of U. */
+int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int nsv,
+                        dpca_model **out);
+int dpca_model_info(dpca_model *m, long long *N, int *nsv, long long *bytes);
+/* Scores (n x nsv) of n new samples given SNP-major like a .bed body: N rows
+ * of ceil(n/4) bytes, same SNP order as the training matrix; missing calls
+ * contribute 0 (mean imputation with the training means). */
+int dpca_model_project(dpca_model *m, const uint8_t *packed, int n, double *scores);
+void dpca_model_free(dpca_model *m);
+
+typedef struct dpca_server dpca_server;
+typedef struct dpca_server_params {
+    int device;
+    long long hbm_budget;   /* bytes the server may hold on the device       */
+    int batch_capacity;     /* samples per projection batch                  */
+    int workers;            /* concurrent cohort-PCA jobs (host threads)     */
+} dpca_server_params;
+
+enum { DPCA_TICKET_PROJECT = 1, DPCA_TICKET_JOB = 2 };
+
+typedef struct dpca_ticket_info {
+    int kind, status;       /* status 2 = done, < 0 error code               */
+    long long n_out;        /* values written (n x nsv scores, or nsv sigma) */
+    double t_submit, t_start, t_done;   /* seconds since server creation     */
+    long long batch;        /* projection: batch id                          */
+    int batch_samples;      /* projection: samples in that batch             */
+    long long footprint;    /* job: admitted HBM bytes                       */
+} dpca_ticket_info;
+
+typedef struct dpca_server_counters {
+    long long batches, batch_samples, projections_done, jobs_done, jobs_rejected,
+        budget_free;
+} dpca_server_counters;
+
+int dpca_server_params_init(dpca_server_params *p);
+/* model may be NULL (jobs only).  Requests that arrive while a batch is on
+ * the device join the next batch (continuous batching); the batch's grouped
+ * projection is a captured HIP graph replayed per batch.  Jobs are admitted
+ * first-fit by dpca_footprint against the free budget and run concurrently
+ * on their own streams. */
+int dpca_server_create(const dpca_server_params *p, dpca_model *model, dpca_server **out);
+/* The payload is read when its batch runs: keep it alive until the wait. */
+int dpca_server_submit_project(dpca_server *s, const uint8_t *packed, int n, long long *ticket);
+/* A cohort-PCA job on a synthetic cohort generated in HBM. */
+int dpca_server_submit_job(dpca_server *s, const dpca_gen_spec *spec, const dpca_params *p,
+                           long long *ticket);
+/* Blocks until the ticket is done; copies up to cap results into out. */
+int dpca_server_wait(dpca_server *s, long long ticket, double *out, long long cap,
+                     dpca_ticket_info *info);
+int dpca_server_stats(dpca_server *s, dpca_server_counters *st);
+void dpca_server_destroy(dpca_server *s);
+
 #ifdef __cplusplus
 }
 #endif

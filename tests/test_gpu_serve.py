@@ -1,0 +1,111 @@
+"""GPU checks of loadings, projection and the serving loop.
+
+Projection arithmetic is pinned against the oracle: scores = z^T W with
+W = Z U diag(1/sigma^2) and z standardised by the training table (dense
+numpy from the oracle's decode).  Tolerances: FP64 form 1e-12 of the score
+scale, mixed form 1e-5.  Serving: a request's scores are bitwise the same
+alone and inside a mixed batch, and a cohort job's sigma is bitwise the same
+alone and run concurrently with other jobs and projections.
+"""
+import numpy as np
+import pytest
+
+import distpca_amd as D
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+M, N = 800, 20000
+
+
+def _train(precision):
+    spec = D.gen_spec(M, N, npop=3, fst=0.05, missing=0.01, seed=21)
+    p = D.Params(nsv=4, nrhs=10, power=3, crit=1, tol=1e-12, miter=80, bsize=5000,
+                 precision=precision)
+    pca = D.PCA.synthetic(spec, p)
+    sigma, rhs, rep = pca.run()
+    return pca, sigma, rhs
+
+
+def _oracle_scores(zval, packed_new, n, W):
+    codes = np.stack([(packed_new[:, i >> 2] >> (2 * (i & 3))) & 3 for i in range(n)], axis=1)
+    z = zval[np.arange(N)[:, None], codes]      # N x n, training standardisation
+    return z.T @ W
+
+
+@pytest.mark.parametrize("precision", [D.FP64, D.MIXED])
+def test_projection_matches_oracle_and_training_rows(precision):
+    print(f"[gpu] projection precision {precision}", flush=True)
+    pca, sigma, rhs = _train(precision)
+    packed = pca.packed()
+    s, c, zval = pca.snp_stats()
+    model = D.Model(pca, rhs, sigma, 4)
+    Z = O.dense_z(packed.reshape(-1), N, M)
+    W = (Z @ rhs[:, :4]) / sigma[:4] ** 2
+    # training samples project onto their own rows of U (converged PCs 1-2)
+    got = model.project(packed, M)
+    ref = Z.T @ W
+    tol = 1e-12 if precision == D.FP64 else 1e-5
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+    assert np.abs(got[:, :2] - rhs[:, :2]).max() < 1e-6 * np.abs(rhs[:, :2]).max() + (
+        0 if precision == D.FP64 else 1e-5)
+    # new samples from the same populations, odd count (ragged last byte)
+    new = D.PCA.synthetic(D.gen_spec(37, N, npop=3, fst=0.05, missing=0.02, seed=99),
+                          D.Params(nsv=2, nrhs=4))
+    pn = new.packed()
+    gn = model.project(pn, 37)
+    rn = _oracle_scores(zval, pn, 37, W)
+    assert np.abs(gn - rn).max() <= tol * np.abs(rn).max()
+    model.close()
+    pca.close()
+    new.close()
+
+
+@pytest.mark.parametrize("precision", [D.FP64, D.MIXED])
+def test_server_same_alone_and_in_mix(precision):
+    print(f"[gpu] server precision {precision}", flush=True)
+    pca, sigma, rhs = _train(precision)
+    model = D.Model(pca, rhs, sigma, 4)
+    new = D.PCA.synthetic(D.gen_spec(600, N, npop=3, fst=0.05, missing=0.02, seed=7),
+                          D.Params(nsv=2, nrhs=4))
+    pool = new.packed()
+    srv = D.Server(model, hbm_budget=8 << 30, batch_capacity=1024, workers=2)
+    reqs = {n: np.ascontiguousarray(pool[:, :(n + 3) // 4]) for n in (1, 5, 130, 400)}
+    alone = {}
+    for n, pay in reqs.items():
+        t = srv.submit_project(pay, n)
+        alone[n], info = srv.wait(t)
+        assert info["batch_samples"] >= n
+    jspec = D.gen_spec(500, 8000, npop=3, fst=0.05, missing=0.01, seed=5)
+    jpar = D.Params(nsv=3, nrhs=6, power=2, crit=1, tol=1e-8, miter=30, precision=precision)
+    tj = srv.submit_job(jspec, jpar)
+    sig_alone, _ = srv.wait(tj)
+    # now everything at once: two jobs and all requests in flight together
+    tjs = [srv.submit_job(jspec, jpar), srv.submit_job(
+        D.gen_spec(700, 12000, npop=2, fst=0.05, missing=0.01, seed=6), jpar)]
+    ts = {n: srv.submit_project(pay, n) for n, pay in reqs.items()}
+    batches = set()
+    for n, t in ts.items():
+        sc, info = srv.wait(t)
+        batches.add(info["batch"])
+        assert np.array_equal(sc, alone[n]), n
+        assert np.abs(sc - model.project(reqs[n], n)).max() <= 1e-12 * np.abs(sc).max()
+    sig_mix, jinfo = srv.wait(tjs[0])
+    srv.wait(tjs[1])
+    assert np.array_equal(sig_mix, sig_alone)
+    assert jinfo["footprint"] > 0
+    assert len(batches) < len(reqs)  # several requests shared a batch
+    cnt = srv.counters()
+    assert cnt["jobs_done"] == 3 and cnt["projections_done"] == 8
+    srv.close()
+    model.close()
+    pca.close()
+    new.close()
+
+
+def test_server_rejects_job_beyond_budget():
+    srv = D.Server(None, hbm_budget=64 << 20, workers=1)
+    with pytest.raises(RuntimeError, match="exceeds the HBM budget"):
+        srv.submit_job(D.gen_spec(20000, 200000, npop=2), D.Params(nsv=2, nrhs=4))
+    assert srv.counters()["jobs_rejected"] == 1
+    srv.close()
