@@ -70,6 +70,118 @@ def cpu_baseline(pca, M, N, k, target_s=15.0):
                       f"nrhs {k}) of the same matrix: {dt:.2f} s; bsize {min(rows, bs)}"}
 
 
+def serve_bench(a, D):
+    """Serving workload: seeded Poisson arrivals; 70% projections of 1-4,096
+    new samples (log-uniform) onto a resident 1,000,000-SNP nsv-20 loading
+    set (5-population, Fst 0.01 generator, 20,000 training samples); 30%
+    cohort-PCA jobs on 2,000-50,000 samples x 100,000-1,000,000 SNPs
+    (log-uniform), nsv 10, nrhs 20.  Same trace for the mixed and FP64 forms."""
+    import threading
+    rng = np.random.default_rng(a.seed)
+    Ns = 1_000_000
+    n_req = int(a.rate * a.duration)
+    gaps = rng.exponential(1.0 / a.rate, n_req)
+    arrivals = np.cumsum(gaps)
+    kinds = rng.random(n_req) < 0.7
+    sizes = np.unique(np.round(np.exp(rng.uniform(0, np.log(4096), 16))).astype(int))
+    req_size = rng.choice(sizes, n_req)
+    job_M = np.round(np.exp(rng.uniform(np.log(2000), np.log(50000), n_req))).astype(int)
+    job_N = np.round(np.exp(rng.uniform(np.log(100000), np.log(1000000), n_req))).astype(int)
+    log(f"serve: {n_req} requests over {arrivals[-1]:.1f} s, "
+        f"{int(kinds.sum())} projections, {int((~kinds).sum())} jobs")
+    new = D.PCA.synthetic(D.gen_spec(4096, Ns, npop=5, fst=0.01, missing=0.01, seed=11),
+                          D.Params(nsv=2, nrhs=4))
+    pool = new.packed()
+    new.close()
+    try:
+        import torch
+        def pinned(x):
+            t = torch.empty(x.shape, dtype=torch.uint8, pin_memory=True)
+            t.numpy()[...] = x
+            return t.numpy(), t
+    except Exception:
+        def pinned(x):
+            return np.ascontiguousarray(x), None
+    payloads = {int(n): pinned(pool[:, :(int(n) + 3) // 4]) for n in sizes}
+    del pool
+    results = {}
+    for form_name, form in (("mixed", D.MIXED), ("fp64", D.FP64)):
+        log(f"serve[{form_name}]: training the loading set (20000 x {Ns})")
+        tr = D.PCA.synthetic(D.gen_spec(20000, Ns, npop=5, fst=0.01, missing=0.01, seed=1),
+                             D.Params(nsv=20, nrhs=40, power=2, crit=1, tol=1e-3,
+                                      precision=form))
+        sig, U, rep = tr.run()
+        model = D.Model(tr, U, sig, 20)
+        tr.close()
+        srv = D.Server(model, hbm_budget=int(200e9), batch_capacity=16384, workers=4)
+        tickets = []
+        lat = {"project": [], "job": []}
+        done_evt = threading.Event()
+
+        def waiter():
+            for kind, t in iter_tickets():
+                _, info = srv.wait(t)
+                lat[kind].append(info["t_done"] - info["t_submit"])
+            done_evt.set()
+
+        q = []
+        cond = threading.Condition()
+
+        def iter_tickets():
+            i = 0
+            while i < n_req:
+                with cond:
+                    while len(q) <= i:
+                        cond.wait()
+                    item = q[i]
+                i += 1
+                yield item
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i in range(n_req):
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            if kinds[i]:
+                pay = payloads[int(req_size[i])][0]
+                t = srv.submit_project(pay, int(req_size[i]))
+                item = ("project", t)
+            else:
+                spec = D.gen_spec(int(job_M[i]), int(job_N[i]), npop=5, fst=0.01,
+                                  missing=0.01, seed=1000 + i)
+                t = srv.submit_job(spec, D.Params(nsv=10, nrhs=20, precision=form))
+                item = ("job", t)
+            with cond:
+                q.append(item)
+                cond.notify_all()
+        th.join()
+        wall = time.perf_counter() - t0
+        cnt = srv.counters()
+        srv.close()
+        model.close()
+        res = {"wall_s": round(wall, 2), "requests_per_s": round(n_req / wall, 3),
+               "jobs_per_s": round(len(lat["job"]) / wall, 3),
+               "batches": cnt["batches"],
+               "mean_batch_samples": round(cnt["batch_samples"] / max(1, cnt["batches"]), 1)}
+        for kind in ("project", "job"):
+            v = np.array(lat[kind]) * 1e3
+            if v.size:
+                res[f"{kind}_p50_ms"] = round(float(np.percentile(v, 50)), 2)
+                res[f"{kind}_p99_ms"] = round(float(np.percentile(v, 99)), 2)
+                res[f"{kind}_n"] = int(v.size)
+        log(f"serve[{form_name}]: {res}")
+        results[form_name] = res
+    out = {"metric": "serving p50/p99 latency and jobs/s, mixed vs FP64",
+           "config": {"workload": "c5 serving", "rate_per_s": a.rate, "duration_s": a.duration,
+                      "seed": a.seed, "projection_sizes": [int(x) for x in sizes],
+                      "loading_set": "20000 x 1000000, nsv 20", "workers": 4,
+                      "batch_capacity": 16384},
+           "mixed": results["mixed"], "fp64": results["fp64"]}
+    print(json.dumps(out), flush=True)
+
+
 def load_traffic(name):
     path = os.path.join(ROOT, "profiles", f"traffic_{name}.json")
     if os.path.exists(path):
@@ -83,7 +195,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS) + ["c5"])
+    ap.add_argument("--rate", type=float, default=5.0, help="c5: arrivals per second")
+    ap.add_argument("--duration", type=float, default=20.0, help="c5: seconds of arrivals")
+    ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--fp64-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -109,6 +224,9 @@ def main():
         sys.exit(2)
 
     import distpca_amd as D
+    if a.workload == "c5":
+        serve_bench(a, D)
+        return
     M, N, npop, fst, admix, miss, seed, nsv, nrhs, power, bsize = WORKLOADS[a.workload]
     spec = D.gen_spec(M, N, npop=npop, fst=fst, admix_frac=admix, missing=miss, seed=seed)
     np_ = (M + 3) // 4
