@@ -93,15 +93,11 @@ def serve_bench(a, D):
                           D.Params(nsv=2, nrhs=4))
     pool = new.packed()
     new.close()
-    try:
-        import torch
-        def pinned(x):
-            t = torch.empty(x.shape, dtype=torch.uint8, pin_memory=True)
-            t.numpy()[...] = x
-            return t.numpy(), t
-    except Exception:
-        def pinned(x):
-            return np.ascontiguousarray(x), None
+    def pinned(x):
+        buf = D.PinnedArray(x.shape)
+        buf.array[...] = x
+        return buf.array, buf
+
     payloads = {int(n): pinned(pool[:, :(int(n) + 3) // 4]) for n in sizes}
     del pool
     results = {}

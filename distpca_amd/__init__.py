@@ -90,7 +90,7 @@ EXPORTS = [
     "dpca_footprint", "dpca_model_from_run", "dpca_model_info", "dpca_model_project",
     "dpca_model_free", "dpca_server_params_init", "dpca_server_create",
     "dpca_server_submit_project", "dpca_server_submit_job", "dpca_server_wait",
-    "dpca_server_stats", "dpca_server_destroy",
+    "dpca_server_stats", "dpca_server_destroy", "dpca_host_alloc", "dpca_host_free",
 ]
 
 _lib = None
@@ -140,6 +140,10 @@ def lib():
     L.dpca_server_stats.argtypes = [vp, P(ServerCounters)]
     L.dpca_server_destroy.argtypes = [vp]
     L.dpca_server_destroy.restype = None
+    L.dpca_host_alloc.argtypes = [i64]
+    L.dpca_host_alloc.restype = vp
+    L.dpca_host_free.argtypes = [vp]
+    L.dpca_host_free.restype = None
     _lib = L
     return L
 
@@ -411,6 +415,30 @@ class Server:
         if self._h:
             lib().dpca_server_destroy(self._h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class PinnedArray:
+    """A page-locked host uint8 buffer (dpca_host_alloc) viewed as numpy."""
+
+    def __init__(self, shape):
+        n = int(np.prod(shape))
+        self._p = lib().dpca_host_alloc(n)
+        if not self._p:
+            raise RuntimeError(lib().dpca_last_error().decode())
+        buf = (ctypes.c_uint8 * n).from_address(self._p)
+        self.array = np.frombuffer(buf, dtype=np.uint8).reshape(shape)
+
+    def close(self):
+        if self._p:
+            self.array = None
+            lib().dpca_host_free(self._p)
+            self._p = None
 
     def __del__(self):
         try:

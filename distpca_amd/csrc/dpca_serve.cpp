@@ -534,6 +534,21 @@ int dpca_server_stats(dpca_server *s, dpca_server_counters *st)
     return 0;
 }
 
+void *dpca_host_alloc(long long bytes)
+{
+    void *p = nullptr;
+    if (bytes <= 0 || hipHostMalloc(&p, (size_t)bytes) != hipSuccess) {
+        set_error("dpca_host_alloc(%lld) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+void dpca_host_free(void *p)
+{
+    if (p) (void)hipHostFree(p);
+}
+
 void dpca_server_destroy(dpca_server *s)
 {
     if (!s) return;

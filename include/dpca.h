@@ -227,6 +227,9 @@ int dpca_server_wait(dpca_server *s, long long ticket, double *out, long long ca
                      dpca_ticket_info *info);
 int dpca_server_stats(dpca_server *s, dpca_server_counters *st);
 void dpca_server_destroy(dpca_server *s);
+/* Page-locked host buffers for request payloads (full-speed H2D). */
+void *dpca_host_alloc(long long bytes);
+void dpca_host_free(void *p);
 
 #ifdef __cplusplus
 }
