@@ -174,12 +174,13 @@ class Params:
     precision: int = FP64
     device: int = 0
     device_cap: int = 0
+    strict_passes: int = 1
 
     def c(self) -> _Params:
         p = _Params()
         lib().dpca_params_init(ctypes.byref(p))
         for f in ("nsv", "nrhs", "power", "crit", "tol", "bsize", "miter", "verbose",
-                  "fwrite", "memory", "precision", "device", "device_cap"):
+                  "fwrite", "memory", "precision", "device", "device_cap", "strict_passes"):
             setattr(p, f, getattr(self, f))
         p.prefix = self.prefix.encode()
         return p

@@ -5,7 +5,9 @@
 // reference: -bfile -nsv -nrhs -power -crit -tol -bsize -miter -verbose
 // -fwrite -prefix -memory -fullSVD -about.  Extra flags of this build:
 // -precision fp64|mixed, -device <ordinal>, -devcap <GB of HBM for the packed
-// matrix before it streams from host memory>.
+// matrix before it streams from host memory>, -strict 0 (reuse the last pass
+// of an iteration as the next iteration's first: power passes per iteration
+// instead of power+1).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -128,6 +130,8 @@ int main(int argc, char **argv)
     findarg("power", INT, &p.power, argc, argv);
     findarg("fullSVD", INT, &p.fullsvd, argc, argv);
     findarg("device", INT, &p.device, argc, argv);
+    int strict = 1;
+    if (findarg("strict", INT, &strict, argc, argv)) p.strict_passes = strict != 0;
     double devcap_gb = 0;
     if (findarg("devcap", DOUBLE, &devcap_gb, argc, argv)) p.device_cap = (long long)(devcap_gb * 1e9);
     if (findarg("precision", STR, prec, argc, argv)) {

@@ -760,7 +760,15 @@ static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
                                        c->G));
             }
         } else {
-            if ((r = do_pass(c, c->RHS2, c->ARHS))) return r;
+            if (ii > 0 && !p.strict_passes) {
+                // A A^T (Q X) = (A A^T Q) X: the previous iteration's last
+                // pass, rotated by the Ritz vectors, replaces the first pass.
+                Stage s(c, T_MM);
+                DPCA_CHECK(launch_apply(c->st, c->ARHS, c->X, c->Mpad, c->kp, c->tmp));
+                std::swap(c->ARHS, c->tmp);
+            } else if ((r = do_pass(c, c->RHS2, c->ARHS))) {
+                return r;
+            }
             for (int kk = 0; kk < p.power - 1; kk++) {
                 if ((r = orthonormalise(c, c->ARHS))) return r;
                 if ((r = do_pass(c, c->Q, c->ARHS))) return r;

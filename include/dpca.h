@@ -43,7 +43,11 @@ typedef struct dpca_params {
                                no cap); above it the matrix streams from
                                pinned host memory in bsize-SNP blocks        */
     int strict_passes;      /* 1 (default): power+1 data passes per outer
-                               iteration exactly as the reference            */
+                               iteration exactly as the reference; 0: from
+                               the second iteration on, the first pass is
+                               (A A^T Q) X of the previous iteration (same
+                               mathematics, one pass fewer per iteration;
+                               out-of-core variant only)                     */
 } dpca_params;
 
 /* Timers and counters of one run.  The t_* fields are the reference's

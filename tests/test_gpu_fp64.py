@@ -180,6 +180,26 @@ def test_run_c1_synthetic(c1_golden):
     assert O.principal_angles(rhs[:, :2], c1_golden["rhs"][:, :2]).max() < 1e-8
 
 
+@pytest.mark.parametrize("precision", [D.FP64, D.MIXED])
+def test_pass_saving_schedule(c1_golden, precision):
+    # A A^T (Q X) = (A A^T Q) X: from the second iteration on the first pass
+    # is the previous last pass rotated by the Ritz vectors.  Same mathematics
+    # as the reference's schedule, so the converged components must agree
+    # with the oracle's strict run.
+    _banner(f"run c1 with one pass saved per iteration, precision {precision}")
+    spec = D.gen_spec(1024, 50000, npop=3, fst=0.05, missing=0.01, seed=0)
+    p = D.Params(nsv=10, nrhs=20, power=4, crit=1, tol=1e-6, miter=20, bsize=4096,
+                 precision=precision, strict_passes=0)
+    with D.PCA.synthetic(spec, p) as pca:
+        sigma, rhs, rep = pca.run()
+    its = rep["total_its"]
+    assert rep["passes"] == 5 + 4 * (its - 1)
+    ev = c1_golden["sigma"][:2] ** 2
+    tol_e, tol_a = (1e-10, 1e-8) if precision == D.FP64 else (1e-6, 1e-4)
+    assert np.abs(sigma[:2] ** 2 - ev).max() / ev.min() < tol_e
+    assert O.principal_angles(rhs[:, :2], c1_golden["rhs"][:, :2]).max() < tol_a
+
+
 def test_incore_variant_matches_oracle():
     M, N = 400, 2000
     packed = _packed_random(M, N, seed=11)
