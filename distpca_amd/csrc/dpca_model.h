@@ -25,7 +25,10 @@ namespace dpca {
 // scores[i][j] = sum_s z_si W_sj with z standardised by the training table.
 // `part` holds the split partials (FP64 form: splits x 4*ldb x kp doubles;
 // mixed: nsplit x 4*ldb x kp floats).  Launch parameters are fixed per
-// (model, ldb) so the launches can be captured in a graph.
+// (model, ldb) so the launches can be captured in a graph; the SNP split is
+// fixed per model (batch-invariant reduction order).
+constexpr int64_t kProjSplitRowsF64 = 4096;  // SNPs per split, FP64 form
+constexpr int64_t kProjSplitRowsMx = 2048;   // SNPs per split, mixed form
 struct ProjPlan {
     int64_t ldb = 0;
     int splits = 1;            // FP64 form

@@ -33,22 +33,21 @@ static double mono_s()
         .count();
 }
 
+// The SNP split of the grouped product depends only on the model, never on
+// the batch width: every sample's partial sums then cover the same SNP ranges
+// in the same order whatever batch it lands in, so a request's scores are
+// bitwise the same alone and inside any batch.
 ProjPlan plan_projection(const dpca_model *m, int64_t ldb)
 {
     ProjPlan pl;
     pl.ldb = ldb;
     const int64_t samples = 4 * ldb;
     if (m->precision == DPCA_FP64) {
-        const int64_t tiles = ldb / 64;
-        int64_t sp = (2048 + tiles - 1) / tiles;
-        pl.splits = (int)std::max<int64_t>(1, std::min<int64_t>(sp, m->Npad / 256));
+        pl.splits = (int)std::max<int64_t>(1, (m->Npad + kProjSplitRowsF64 - 1) / kProjSplitRowsF64);
         pl.part_bytes = (size_t)pl.splits * samples * m->kp * sizeof(double);
     } else {
         pl.sa_tiles = (ldb % 256 == 0) ? 8 : 2;
-        const int64_t tiles = ldb / (pl.sa_tiles * 32);
-        int64_t ns = std::max<int64_t>((1024 + tiles - 1) / tiles, (m->Npad + 16383) / 16384);
-        ns = std::min<int64_t>(ns, m->Npad / 128);
-        pl.rps = round_up((m->Npad + ns - 1) / ns, 128);
+        pl.rps = std::min<int64_t>(kProjSplitRowsMx, m->Npad);
         pl.nsplit = (int)((m->Npad + pl.rps - 1) / pl.rps);
         pl.part_bytes = (size_t)pl.nsplit * samples * m->kp * sizeof(float);
     }

@@ -89,7 +89,7 @@ def test_server_same_alone_and_in_mix(precision):
         sc, info = srv.wait(t)
         batches.add(info["batch"])
         assert np.array_equal(sc, alone[n]), n
-        assert np.abs(sc - model.project(reqs[n], n)).max() <= 1e-12 * np.abs(sc).max()
+        assert np.array_equal(sc, model.project(reqs[n], n)), n
     sig_mix, jinfo = srv.wait(tjs[0])
     srv.wait(tjs[1])
     assert np.array_equal(sig_mix, sig_alone)
@@ -101,6 +101,28 @@ def test_server_same_alone_and_in_mix(precision):
     model.close()
     pca.close()
     new.close()
+
+
+@pytest.mark.parametrize("precision", [D.FP64, D.MIXED])
+def test_projection_batch_invariant(precision):
+    # a loading set wide enough (200,000 SNPs) that batch classes of 256 to
+    # 4,096 samples would pick different SNP splits if the split followed the
+    # batch width; scores must be bitwise the same whatever the batch size
+    print(f"[gpu] projection batch invariance precision {precision}", flush=True)
+    spec = D.gen_spec(600, 200000, npop=3, fst=0.05, missing=0.01, seed=31)
+    p = D.Params(nsv=3, nrhs=6, power=2, crit=1, tol=1e-6, miter=10, precision=precision)
+    with D.PCA.synthetic(spec, p) as pca:
+        sigma, rhs, _ = pca.run()
+        model = D.Model(pca, rhs, sigma, 3)
+    new = D.PCA.synthetic(D.gen_spec(4000, 200000, npop=3, fst=0.05, missing=0.02, seed=32),
+                          D.Params(nsv=2, nrhs=4))
+    pool = new.packed()
+    new.close()
+    full = model.project(np.ascontiguousarray(pool[:, :1000]), 4000)
+    for n in (1, 7, 300, 1100):
+        part = model.project(np.ascontiguousarray(pool[:, :(n + 3) // 4]), n)
+        assert np.array_equal(part, full[:n]), n
+    model.close()
 
 
 def test_server_rejects_job_beyond_budget():
