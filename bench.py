@@ -26,7 +26,9 @@ WORKLOADS = {
     "c1": (1024, 50000, 3, 0.05, 0.0, 0.01, 0, 10, 20, 4, 4096),
     "c2": (20000, 500000, 5, 0.01, 0.0, 0.01, 1, 10, 20, 4, 16384),
     "c3": (100000, 1000000, 8, 0.005, 0.2, 0.02, 2, 20, 40, 6, 0),
-    # out-of-core: pinned host memory, 16 GB device cap, ~512 MB blocks
+    # out-of-core: host memory (pinned, or pageable staged through pinned
+    # blocks when the driver cannot pin it all), 16 GB device cap, ~512 MB
+    # blocks; the second is the same pipeline at 1/8 of the SNPs (2 GB cap)
     "c4": (400000, 2000000, 10, 0.005, 0.0, 0.01, 3, 20, 40, 4, 65536),
     "c4s": (400000, 250000, 10, 0.005, 0.0, 0.01, 3, 20, 40, 4, 65536),
 }
@@ -254,7 +256,10 @@ def main():
     stream = None
     if cap:
         compute = ms[1] + ms[2] + ms[6]
-        stream = {"device_cap_bytes": cap, "h2d_ms_per_pass": round(ms[5], 2),
+        stream = {"device_cap_bytes": cap,
+                  "host_memory": {0: "pinned", 1: "pageable, staged through pinned blocks",
+                                  2: ".bed file"}.get(int(ms[7]), "?"),
+                  "h2d_ms_per_pass": round(ms[5], 2),
                   "h2d_GBps_copy_engine": round(packed_bytes / (ms[5] * 1e-3) / 1e9, 2),
                   "h2d_GBps_end_to_end": round(value, 2),
                   "compute_ms_per_pass": round(compute, 2),

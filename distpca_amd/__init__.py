@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(_HERE, "libdpca.so")
 EXE_PATH = os.path.join(_HERE, "DistPCA.exe")
 
 FP64, MIXED = 0, 1
+HOST_PINNED, HOST_PAGEABLE, HOST_FILE = 0, 1, 2
 
 
 class _Params(ctypes.Structure):
@@ -30,7 +31,7 @@ class _Params(ctypes.Structure):
         ("fullsvd", ctypes.c_int), ("memory", ctypes.c_double),
         ("prefix", ctypes.c_char * 1024), ("precision", ctypes.c_int),
         ("device", ctypes.c_int), ("device_cap", ctypes.c_longlong),
-        ("strict_passes", ctypes.c_int),
+        ("strict_passes", ctypes.c_int), ("host_mode", ctypes.c_int),
     ]
 
 
@@ -45,6 +46,7 @@ class Report(ctypes.Structure):
         ("t_gs", ctypes.c_double), ("t_rr", ctypes.c_double), ("t_other", ctypes.c_double),
         ("wall", ctypes.c_double), ("last_crit", ctypes.c_double),
         ("t_h2d", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
+        ("host_mode", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -175,12 +177,13 @@ class Params:
     device: int = 0
     device_cap: int = 0
     strict_passes: int = 1
+    host_mode: int = HOST_PINNED
 
     def c(self) -> _Params:
         p = _Params()
         lib().dpca_params_init(ctypes.byref(p))
         for f in ("nsv", "nrhs", "power", "crit", "tol", "bsize", "miter", "verbose",
-                  "fwrite", "memory", "precision", "device", "device_cap", "strict_passes"):
+                  "fwrite", "memory", "precision", "device", "device_cap", "strict_passes", "host_mode"):
             setattr(p, f, getattr(self, f))
         p.prefix = self.prefix.encode()
         return p
@@ -291,9 +294,10 @@ class PCA:
 
     def bench_pass(self, Q: np.ndarray, npass: int):
         """Event-timed passes: (ms per pass, ms Z Q kernel, ms Z^T T kernel, ms other,
-        host wall ms per pass, host-to-device copy ms per pass, small-kernel ms)."""
+        host wall ms per pass, host-to-device copy ms per pass, small-kernel ms,
+        host mode of a streamed matrix or -1)."""
         Q = np.ascontiguousarray(Q, dtype=np.float64)
-        ms = np.zeros(7, np.float64)
+        ms = np.zeros(8, np.float64)
         _check(lib().dpca_bench_pass(self._h, _ptr(Q), npass, _ptr(ms)))
         return ms
 

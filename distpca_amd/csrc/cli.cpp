@@ -7,7 +7,8 @@
 // -precision fp64|mixed, -device <ordinal>, -devcap <GB of HBM for the packed
 // matrix before it streams from host memory>, -strict 0 (reuse the last pass
 // of an iteration as the next iteration's first: power passes per iteration
-// instead of power+1).
+// instead of power+1), -hostmem pinned|pageable|file (where a streamed matrix
+// lives between passes; file re-reads the .bed every pass).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -134,6 +135,19 @@ int main(int argc, char **argv)
     if (findarg("strict", INT, &strict, argc, argv)) p.strict_passes = strict != 0;
     double devcap_gb = 0;
     if (findarg("devcap", DOUBLE, &devcap_gb, argc, argv)) p.device_cap = (long long)(devcap_gb * 1e9);
+    char hostmem[1024] = "";
+    if (findarg("hostmem", STR, hostmem, argc, argv)) {
+        if (!strcmp(hostmem, "pinned"))
+            p.host_mode = DPCA_HOST_PINNED;
+        else if (!strcmp(hostmem, "pageable"))
+            p.host_mode = DPCA_HOST_PAGEABLE;
+        else if (!strcmp(hostmem, "file"))
+            p.host_mode = DPCA_HOST_FILE;
+        else {
+            printf("\nUnknown -hostmem '%s' (pinned | pageable | file)\n", hostmem);
+            return 1;
+        }
+    }
     if (findarg("precision", STR, prec, argc, argv)) {
         if (!strcmp(prec, "mixed"))
             p.precision = DPCA_MIXED;

@@ -166,9 +166,14 @@ struct dpca_ctx {
     float *Tpart = nullptr, *Apart = nullptr;
     int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 8;
     int64_t seg_bytes = 0, rps = 0;
-    // streaming from pinned host memory (matrix above device_cap)
+    // streaming from host memory or the .bed file (matrix above device_cap)
     bool streamed = false;
-    uint8_t *hpacked = nullptr;       // pinned, Npad x ldp, device layout
+    int hmode = DPCA_HOST_PINNED;     // where the streamed matrix lives
+    uint8_t *hpacked = nullptr;       // host copy, Npad x ldp, device layout
+    uint8_t *bounce[2] = {nullptr, nullptr};  // pinned staging (pageable / file)
+    uint8_t *draw[2] = {nullptr, nullptr};    // file mode: raw .bed rows on the device
+    hipEvent_t ev_bounce[2] = {nullptr, nullptr};  // last copy out of bounce[b]
+    int fd = -1;                      // file mode: the .bed file
     uint8_t *dbuf[2] = {nullptr, nullptr};
     int64_t chunk_rows = 0;           // rows per streamed block (multiple of 1024)
     int64_t block_rows = 0;           // rows the per-block buffers are sized for
@@ -262,10 +267,19 @@ static void free_ctx(dpca_ctx *c)
         if (p) (void)hipFree(p);
     for (int b = 0; b < 2; b++) {
         if (c->dbuf[b]) (void)hipFree(c->dbuf[b]);
+        if (c->draw[b]) (void)hipFree(c->draw[b]);
+        if (c->bounce[b]) (void)hipHostFree(c->bounce[b]);
         if (c->ev_copied[b]) (void)hipEventDestroy(c->ev_copied[b]);
         if (c->ev_free[b]) (void)hipEventDestroy(c->ev_free[b]);
+        if (c->ev_bounce[b]) (void)hipEventDestroy(c->ev_bounce[b]);
     }
-    if (c->hpacked) (void)hipHostFree(c->hpacked);
+    if (c->hpacked) {
+        if (c->hmode == DPCA_HOST_PINNED)
+            (void)hipHostFree(c->hpacked);
+        else
+            free(c->hpacked);
+    }
+    if (c->fd >= 0) close(c->fd);
     if (c->cst) (void)hipStreamDestroy(c->cst);
     for (auto &tm : c->pending) {
         if (tm.a) (void)hipEventDestroy(tm.a);
@@ -420,7 +434,7 @@ static int compute_stats(dpca_ctx *c)
     return r;
 }
 
-static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out)
+static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool from_file = false)
 {
     *out = nullptr;
     if (!p) {
@@ -467,23 +481,58 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out)
             free_ctx(c);
             return -2;
         }
+        c->hmode = c->p.host_mode;
+        if (c->hmode < DPCA_HOST_PINNED || c->hmode > DPCA_HOST_FILE ||
+            (c->hmode == DPCA_HOST_FILE && !from_file)) {
+            set_error("host_mode %d is not available here (file mode needs dpca_open)", c->hmode);
+            free_ctx(c);
+            return -1;
+        }
         for (int b = 0; b < 2; b++) {
             if ((r = dalloc(&c->dbuf[b], (size_t)(c->chunk_rows * c->ldp)))) {
                 free_ctx(c);
                 return r;
             }
             if (hipEventCreateWithFlags(&c->ev_copied[b], hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&c->ev_free[b], hipEventDisableTiming) != hipSuccess) {
+                hipEventCreateWithFlags(&c->ev_free[b], hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&c->ev_bounce[b], hipEventDisableTiming) != hipSuccess) {
                 set_error("hipEventCreate failed");
                 free_ctx(c);
                 return -2;
             }
             (void)hipEventRecord(c->ev_free[b], c->st);
+            (void)hipEventRecord(c->ev_bounce[b], c->st);
         }
-        if (hipHostMalloc((void **)&c->hpacked, (size_t)bytes) != hipSuccess) {
-            set_error("hipHostMalloc(%lld bytes) for the streamed matrix failed", (long long)bytes);
+        // Pinned host memory is limited by the driver (typically half the
+        // RAM); above that the matrix stays pageable and each block is staged
+        // through a pinned buffer.
+        if (c->hmode == DPCA_HOST_PINNED &&
+            hipHostMalloc((void **)&c->hpacked, (size_t)bytes) != hipSuccess) {
+            c->hpacked = nullptr;
+            (void)hipGetLastError();
+            c->hmode = DPCA_HOST_PAGEABLE;
+        }
+        if (c->hmode == DPCA_HOST_PAGEABLE && !(c->hpacked = (uint8_t *)malloc((size_t)bytes))) {
+            set_error("malloc(%lld bytes) for the streamed matrix failed", (long long)bytes);
             free_ctx(c);
             return -3;
+        }
+        if (c->hmode != DPCA_HOST_PINNED) {
+            const int64_t row_bytes = (c->hmode == DPCA_HOST_FILE) ? c->np : c->ldp;
+            for (int b = 0; b < 2; b++) {
+                if (hipHostMalloc((void **)&c->bounce[b], (size_t)(c->chunk_rows * row_bytes)) !=
+                    hipSuccess) {
+                    set_error("hipHostMalloc of a %lld-byte staging buffer failed",
+                              (long long)(c->chunk_rows * row_bytes));
+                    free_ctx(c);
+                    return -3;
+                }
+                if (c->hmode == DPCA_HOST_FILE &&
+                    (r = dalloc(&c->draw[b], (size_t)(c->chunk_rows * c->np)))) {
+                    free_ctx(c);
+                    return r;
+                }
+            }
         }
     }
     if ((r = alloc_work(c))) {
@@ -519,18 +568,123 @@ static void host_repack_rows(const uint8_t *raw, int64_t n, int64_t np, int64_t 
     for (auto &x : th) x.join();
 }
 
+// Host copy split over threads (pageable -> pinned staging and back).
+static void par_copy(uint8_t *dst, const uint8_t *src, size_t n)
+{
+    const int T = host_threads();
+    if (n < (16u << 20) || T == 1) {
+        memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = ((n + T - 1) / T + 4095) & ~(size_t)4095;
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; t++) {
+        const size_t a = (size_t)t * per;
+        if (a >= n) break;
+        const size_t len = std::min(per, n - a);
+        th.emplace_back([=] { memcpy(dst + a, src + a, len); });
+    }
+    for (auto &x : th) x.join();
+}
+
+// Reads n bytes at offset off of fd into dst, split over threads.
+static int par_pread(int fd, uint8_t *dst, size_t n, off_t off)
+{
+    const int T = host_threads();
+    const size_t per = std::max<size_t>(((n + T - 1) / T + 4095) & ~(size_t)4095, 1 << 20);
+    std::vector<std::thread> th;
+    std::vector<int> bad(T, 0);
+    for (int t = 0; t < T; t++) {
+        const size_t a = (size_t)t * per;
+        if (a >= n) break;
+        const size_t len = std::min(per, n - a);
+        th.emplace_back([=, &bad] {
+            size_t done = 0;
+            while (done < len) {
+                const ssize_t got = pread(fd, dst + a + done, len - done, off + (off_t)(a + done));
+                if (got <= 0) {
+                    if (got < 0 && errno == EINTR) continue;
+                    bad[t] = 1;
+                    return;
+                }
+                done += (size_t)got;
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    for (int b : bad)
+        if (b) return -1;
+    return 0;
+}
+
+// Enqueues on stream s the copy of streamed rows [r0, r0 + rows) (a block of
+// at most chunk_rows) into device buffer dst in the device layout.  Pinned
+// host copy: one H2D copy.  Pageable copy: the rows are first copied into
+// pinned staging buffer b; .bed file: the raw rows are read into staging
+// buffer b, copied to the device and repacked there (k_repack, padding rows
+// filled).  The host waits until the previous copy out of staging buffer b
+// has finished before it refills it.
+static int fetch_block(dpca_ctx *c, int b, int64_t r0, int64_t rows, uint8_t *dst, hipStream_t s)
+{
+    if (c->hmode == DPCA_HOST_PINNED) {
+        DPCA_CHECK(hipMemcpyAsync(dst, c->hpacked + r0 * c->ldp, (size_t)(rows * c->ldp),
+                                  hipMemcpyHostToDevice, s));
+        c->bytes_h2d += (double)(rows * c->ldp);
+        return 0;
+    }
+    DPCA_CHECK(hipEventSynchronize(c->ev_bounce[b]));
+    if (c->hmode == DPCA_HOST_PAGEABLE) {
+        par_copy(c->bounce[b], c->hpacked + r0 * c->ldp, (size_t)(rows * c->ldp));
+        DPCA_CHECK(hipMemcpyAsync(dst, c->bounce[b], (size_t)(rows * c->ldp),
+                                  hipMemcpyHostToDevice, s));
+        c->bytes_h2d += (double)(rows * c->ldp);
+    } else {
+        const int64_t real = std::max<int64_t>(0, std::min(rows, c->N - r0));
+        if (real > 0) {
+            if (par_pread(c->fd, c->bounce[b], (size_t)(real * c->np), (off_t)(3 + r0 * c->np))) {
+                set_error("short read of SNPs [%lld, %lld) from the .bed file", (long long)r0,
+                          (long long)(r0 + real));
+                return -1;
+            }
+            DPCA_CHECK(hipMemcpyAsync(c->draw[b], c->bounce[b], (size_t)(real * c->np),
+                                      hipMemcpyHostToDevice, s));
+            DPCA_CHECK(launch_repack(s, c->draw[b], real, c->np, dst, c->ldp, c->M));
+            c->bytes_h2d += (double)(real * c->np);
+        }
+        if (rows > real)
+            DPCA_CHECK(launch_fill_u8(s, dst + real * c->ldp, (rows - real) * c->ldp, kPadByte));
+    }
+    DPCA_CHECK(hipEventRecord(c->ev_bounce[b], s));
+    return 0;
+}
+
 // Streamed matrix: per-SNP statistics without a pass (one copy sweep).
 static int stream_stats_sweep(dpca_ctx *c)
 {
     for (int64_t r0 = 0; r0 < c->Npad; r0 += c->chunk_rows) {
         const int64_t rows = std::min(c->chunk_rows, c->Npad - r0);
-        DPCA_CHECK(hipMemcpyAsync(c->dbuf[0], c->hpacked + r0 * c->ldp, (size_t)(rows * c->ldp),
-                                  hipMemcpyHostToDevice, c->st));
-        int r = block_stats(c, c->dbuf[0], r0, rows);
+        int r = fetch_block(c, 0, r0, rows, c->dbuf[0], c->st);
+        if (!r) r = block_stats(c, c->dbuf[0], r0, rows);
         if (r) return r;
         DPCA_CHECK(hipStreamSynchronize(c->st));
     }
     c->stats_done = true;
+    return 0;
+}
+
+// Device-layout block rows [r0, r0 + rows) (from the device) into the host
+// copy of a streamed matrix, through staging buffer 0 when it is pageable.
+static int park_block(dpca_ctx *c, const uint8_t *src, int64_t r0, int64_t rows)
+{
+    const size_t n = (size_t)(rows * c->ldp);
+    if (c->hmode == DPCA_HOST_PINNED) {
+        DPCA_CHECK(hipMemcpyAsync(c->hpacked + r0 * c->ldp, src, n, hipMemcpyDeviceToHost, c->st));
+        DPCA_CHECK(hipStreamSynchronize(c->st));
+        return 0;
+    }
+    DPCA_CHECK(hipMemcpyAsync(c->bounce[0], src, n, hipMemcpyDeviceToHost, c->st));
+    DPCA_CHECK(hipStreamSynchronize(c->st));
+    par_copy(c->hpacked + r0 * c->ldp, c->bounce[0], n);
     return 0;
 }
 
@@ -629,7 +783,9 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
 
 // Streamed pass: block j is copied on the copy stream into buffer j&1 while
 // block j-1 is multiplied on the compute stream; events order the reuse of
-// each buffer.  The first streamed pass also computes the per-SNP statistics
+// each buffer.  With a pageable host copy or the .bed file as the source, the
+// host fills staging buffer j&1 while the copy engine and the compute stream
+// work on the two blocks before it.  The first streamed pass also computes the per-SNP statistics
 // of each block before its products (first touch, src/methods.cpp:819-836).
 static int pass_streamed(dpca_ctx *c, const double *Qin, double *out, bool half)
 {
@@ -640,15 +796,13 @@ static int pass_streamed(dpca_ctx *c, const double *Qin, double *out, bool half)
         const int64_t r0 = j * c->chunk_rows;
         const int64_t rows = std::min(c->chunk_rows, c->Npad - r0);
         DPCA_CHECK(hipStreamWaitEvent(c->cst, c->ev_free[b], 0));
+        int r;
         {
             Stage s(c, T_H2D, c->cst);
-            DPCA_CHECK(hipMemcpyAsync(c->dbuf[b], c->hpacked + r0 * c->ldp, (size_t)(rows * c->ldp),
-                                      hipMemcpyHostToDevice, c->cst));
+            if ((r = fetch_block(c, b, r0, rows, c->dbuf[b], c->cst))) return r;
         }
-        c->bytes_h2d += (double)(rows * c->ldp);
         DPCA_CHECK(hipEventRecord(c->ev_copied[b], c->cst));
         DPCA_CHECK(hipStreamWaitEvent(c->st, c->ev_copied[b], 0));
-        int r;
         if (!c->stats_done && (r = block_stats(c, c->dbuf[b], r0, rows))) return r;
         if ((r = pass_block(c, c->dbuf[b], r0, rows, Qin, out, j > 0, half))) return r;
         DPCA_CHECK(hipEventRecord(c->ev_free[b], c->st));
@@ -850,6 +1004,7 @@ static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
         rep->streamed = c->streamed;
         rep->t_h2d = c->t[T_H2D];
         rep->h2d_bytes = c->bytes_h2d;
+        rep->host_mode = c->streamed ? c->hmode : -1;
     }
     return 0;
 }
@@ -923,6 +1078,7 @@ int dpca_params_init(dpca_params *p)
     p->device = 0;
     p->device_cap = 0;
     p->strict_passes = 1;
+    p->host_mode = DPCA_HOST_PINNED;
     return 0;
 }
 
@@ -964,7 +1120,7 @@ int dpca_open(const char *bfile, const dpca_params *p, dpca_ctx **out)
         return -1;
     }
     dpca_ctx *c = nullptr;
-    int r = new_ctx(p, M, N, &c);
+    int r = new_ctx(p, M, N, &c, true);
     if (r) {
         fclose(fb);
         return r;
@@ -972,6 +1128,18 @@ int dpca_open(const char *bfile, const dpca_params *p, dpca_ctx **out)
     c->iids.reserve(M);
     for (auto &l : fam) c->iids.push_back(token(l, 1));
     fseek(fb, 3, SEEK_SET);  // 3-byte magic, not validated (src/distpca.cpp:105)
+    if (c->streamed && c->hmode == DPCA_HOST_FILE) {
+        // every pass reads the file again (src/methods.cpp:795-811)
+        c->fd = dup(fileno(fb));
+        fclose(fb);
+        if (c->fd < 0) {
+            set_error("dup of the .bed descriptor failed");
+            free_ctx(c);
+            return -1;
+        }
+        *out = c;
+        return 0;
+    }
     if (c->streamed) {
         // the matrix stays in pinned host memory in the device row layout
         const int64_t chunk = std::max<int64_t>(1, (256ll << 20) / np);
@@ -1109,10 +1277,7 @@ int dpca_open_synthetic(const dpca_gen_spec *g, const dpca_params *p, dpca_ctx *
             for (int64_t r0 = 0; r0 < c->Npad && e == hipSuccess; r0 += c->chunk_rows) {
                 const int64_t rows = std::min(c->chunk_rows, c->Npad - r0);
                 e = launch_generate(c->st, spec, dthr, dwts, r0, rows, c->dbuf[0], c->ldp);
-                if (e == hipSuccess)
-                    e = hipMemcpyAsync(c->hpacked + r0 * c->ldp, c->dbuf[0], (size_t)(rows * c->ldp),
-                                       hipMemcpyDeviceToHost, c->st);
-                if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+                if (e == hipSuccess && park_block(c, c->dbuf[0], r0, rows)) e = hipErrorUnknown;
             }
         } else if (e == hipSuccess) {
             e = launch_generate(c->st, spec, dthr, dwts, 0, c->Npad, c->packed, c->ldp);
@@ -1227,8 +1392,24 @@ int dpca_decode(dpca_ctx *c, long long snp0, long long rows, uint8_t *out)
             (void)hipFree(d);
             return -3;
         }
-        e = hipMemcpyAsync(src, c->hpacked + snp0 * c->ldp, (size_t)(rows * c->ldp),
-                           hipMemcpyHostToDevice, c->st);
+        if (c->hmode != DPCA_HOST_FILE) {
+            e = hipMemcpyAsync(src, c->hpacked + snp0 * c->ldp, (size_t)(rows * c->ldp),
+                               hipMemcpyHostToDevice, c->st);
+        } else {
+            std::vector<uint8_t> raw((size_t)(rows * c->np));
+            uint8_t *draw = nullptr;
+            if (par_pread(c->fd, raw.data(), raw.size(), (off_t)(3 + snp0 * c->np)) ||
+                dalloc(&draw, raw.size())) {
+                set_error("dpca_decode: reading the .bed file failed");
+                (void)hipFree(d);
+                (void)hipFree(src);
+                return -1;
+            }
+            e = hipMemcpy(draw, raw.data(), raw.size(), hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = launch_repack(c->st, draw, rows, c->np, src, c->ldp, c->M);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+            (void)hipFree(draw);
+        }
         rowsrc = src;
     }
     if (e == hipSuccess) e = launch_decode_codes(c->st, rowsrc, rows, c->ldp, c->M, d);
@@ -1251,7 +1432,12 @@ int dpca_copy_packed(dpca_ctx *c, long long snp0, long long rows, uint8_t *out)
         return -1;
     }
     (void)hipSetDevice(c->dev);
-    if (c->streamed) {
+    if (c->streamed && c->hmode == DPCA_HOST_FILE) {
+        if (par_pread(c->fd, out, (size_t)(rows * c->np), (off_t)(3 + snp0 * c->np))) {
+            set_error("dpca_copy_packed: reading the .bed file failed");
+            return -1;
+        }
+    } else if (c->streamed) {
         for (long long r = 0; r < rows; r++)
             memcpy(out + r * c->np, c->hpacked + (snp0 + r) * c->ldp, (size_t)c->np);
     } else {
@@ -1467,6 +1653,7 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
     ms[4] = hwall * 1e3 / npass;
     ms[5] = kc / npass;
     ms[6] = kd / npass;
+    ms[7] = c->streamed ? c->hmode : -1;
     return 0;
 }
 
@@ -1492,6 +1679,7 @@ int dpca_footprint(int M, long long N, const dpca_params *p, long long *bytes)
         c.chunk_rows = std::min(c.Npad, std::max<int64_t>(1024, target / c.ldp / 1024 * 1024));
         c.block_rows = c.chunk_rows;
         total += 2 * (size_t)(c.chunk_rows * c.ldp);
+        if (c.p.host_mode == DPCA_HOST_FILE) total += 2 * (size_t)(c.chunk_rows * c.np);
     } else {
         total += (size_t)packed;
     }

@@ -48,7 +48,17 @@ typedef struct dpca_params {
                                (A A^T Q) X of the previous iteration (same
                                mathematics, one pass fewer per iteration;
                                out-of-core variant only)                     */
+    int host_mode;          /* where a streamed matrix lives between passes:
+                               DPCA_HOST_PINNED (default; falls back to
+                               DPCA_HOST_PAGEABLE when the pin fails),
+                               DPCA_HOST_PAGEABLE (pageable memory staged
+                               through two pinned block buffers) or
+                               DPCA_HOST_FILE (dpca_open only: every pass
+                               reads the .bed file again, as the reference's
+                               MPI_File_iread_at loop, src/methods.cpp:795-811) */
 } dpca_params;
+
+enum { DPCA_HOST_PINNED = 0, DPCA_HOST_PAGEABLE = 1, DPCA_HOST_FILE = 2 };
 
 /* Timers and counters of one run.  The t_* fields are the reference's
  * TIME_2_* stage timers (src/structures.h:87-88), in seconds; `wall` is the
@@ -67,6 +77,7 @@ typedef struct dpca_report {
     double last_crit;       /* last criterion value (crit 0/2) or count (1)  */
     double t_h2d;           /* streamed: seconds of host-to-device copies    */
     double h2d_bytes;       /* streamed: bytes copied host-to-device         */
+    int host_mode;          /* streamed: the DPCA_HOST_* mode in use         */
 } dpca_report;
 
 /* Defaults of initialize_structure (src/utilities.cpp:195-232). */
@@ -161,7 +172,8 @@ int dpca_syev(const double *B, int k, double *w, double *X, int device);
  * splits, partial reductions), ms[4] host wall clock per pass (launch to
  * stream drain), ms[5] host-to-device copy time per pass (streamed
  * matrices; it overlaps the kernels), ms[6] kernel time of the small pass
- * kernels.  ms must hold 7 values. */
+ * kernels, ms[7] the DPCA_HOST_* mode of a streamed matrix (-1 when
+ * resident).  ms must hold 8 values. */
 int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms);
 
 void dpca_close(dpca_ctx *c);

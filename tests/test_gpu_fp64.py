@@ -276,3 +276,39 @@ def test_streamed_matches_resident(precision):
     assert ra["total_its"] == rb["total_its"]
     assert np.allclose(sa, sb, rtol=1e-10 if precision == D.FP64 else 1e-6)
     assert O.principal_angles(Ua[:, :2], Ub[:, :2]).max() < (1e-8 if precision == D.FP64 else 1e-4)
+
+
+@pytest.mark.parametrize("mode", [D.HOST_PAGEABLE, D.HOST_FILE])
+def test_streamed_host_modes(tmp_path, mode):
+    # the matrix kept pageable (staged through pinned block buffers) or left
+    # in the .bed file and re-read every pass (raw rows repacked on the
+    # device); both must give the resident context's bytes, statistics and
+    # FP64 results
+    _banner(f"streamed host mode {mode}")
+    M, N = 1001, 12345                     # ragged last byte, partial last block
+    pre = str(tmp_path / "hm")
+    spec = D.gen_spec(M, N, npop=3, fst=0.05, missing=0.02, seed=13)
+    D.write_synthetic(pre, spec)
+    base = dict(nsv=3, nrhs=6, power=2, crit=1, tol=1e-10, miter=6, bsize=3000)
+    if mode == D.HOST_PAGEABLE:
+        # device-generated blocks parked in pageable memory through staging
+        with D.PCA.synthetic(spec, D.Params(device_cap=1 << 20, host_mode=mode, **base)) as g, \
+                D.PCA.from_bfile(pre, D.Params(**base)) as res:
+            assert np.array_equal(g.packed(), res.packed())
+    with D.PCA.from_bfile(pre, D.Params(**base)) as res, \
+            D.PCA.from_bfile(pre, D.Params(device_cap=1 << 20, host_mode=mode, **base)) as stm:
+        s1, c1, z1 = res.snp_stats()
+        s2, c2, z2 = stm.snp_stats()
+        assert np.array_equal(s1, s2) and np.array_equal(c1, c2)
+        assert np.array_equal(z1.view(np.int64), z2.view(np.int64))
+        assert np.array_equal(res.packed(), stm.packed())
+        assert np.array_equal(res.decode(N - 7, 7), stm.decode(N - 7, 7))
+        Q = np.linalg.qr(np.random.default_rng(2).standard_normal((M, 6)))[0]
+        a, b = res.pass_(Q), stm.pass_(Q)
+        assert np.abs(a - b).max() / np.abs(a).max() < 1e-12
+        sa, Ua, ra = res.run()
+        sb, Ub, rb = stm.run()
+    assert rb["streamed"] == 1 and rb["host_mode"] == mode
+    assert ra["total_its"] == rb["total_its"]
+    assert np.allclose(sa, sb, rtol=1e-10)
+    assert O.principal_angles(Ua[:, :2], Ub[:, :2]).max() < 1e-8
