@@ -162,9 +162,10 @@ struct dpca_ctx {
            *dots = nullptr;
     int ngparts = 256;
     // mixed form
-    uint4 *ztab = nullptr, *QB = nullptr, *UWB = nullptr;
+    uint4 *ztab = nullptr, *QB = nullptr, *UW8 = nullptr;
+    unsigned *smax = nullptr;         // per (SNP split, column) max |U|, |W|
     float *Tpart = nullptr, *Apart = nullptr;
-    int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 8;
+    int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 4;
     int64_t seg_bytes = 0, rps = 0;
     // streaming from host memory or the .bed file (matrix above device_cap)
     bool streamed = false;
@@ -261,8 +262,7 @@ static void free_ctx(dpca_ctx *c)
     (void)hipSetDevice(c->dev);
     void *ptrs[] = {c->packed, c->zval, c->mu, c->dsc, c->ssum, c->sctr, c->Q, c->ARHS,
                     c->RHS2, c->RHS2old, c->tmp, c->T, c->part, c->gpart, c->G, c->Rinv,
-                    c->w, c->X, c->dots, c->flags, c->its, c->ztab, c->QB, c->UWB,
-                    c->Tpart, c->Apart};
+                    c->w, c->X, c->dots, c->flags, c->its, c->ztab, c->QB, c->UW8, c->smax, c->Tpart, c->Apart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int b = 0; b < 2; b++) {
@@ -383,7 +383,8 @@ static int alloc_work(dpca_ctx *c)
         c->st_tiles = (B / 512 * c->nseg >= 512) ? 4 : 2;
         if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 2, 4, 8
         // Z^T T: SNP splits of <= 16384 rows, >= ~1024 workgroups
-        c->sa_tiles = (c->ldp >= 256 * 32) ? 8 : 2;
+        c->sa_tiles = zt_i8_tiles(c->kp, c->ldp);
+        if (const char *e = getenv("DPCA_ZT_TILES")) c->sa_tiles = atoi(e);  // tuning knob: 2, 4
         const int64_t tl = c->ldp / (c->sa_tiles * 32);
         int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 16383) / 16384);
         ns = std::min<int64_t>(ns, B / 128);
@@ -391,7 +392,8 @@ static int alloc_work(dpca_ctx *c)
         c->nsplit = (int)((B + c->rps - 1) / c->rps);
         r |= dalloc(&c->ztab, (size_t)c->Npad * sizeof(uint4));
         r |= dalloc(&c->QB, (size_t)c->Mpad / 8 * c->kp * 2 * sizeof(uint4));
-        r |= dalloc(&c->UWB, (size_t)c->Npad / 4 * c->kp * 2 * sizeof(uint4));
+        r |= dalloc(&c->UW8, (size_t)c->Npad / 8 * 3 * c->kp * sizeof(uint4));
+        r |= dalloc(&c->smax, (size_t)c->nsplit * c->kp * sizeof(unsigned));
         r |= dalloc(&c->Tpart, (size_t)c->nseg * B * c->kp * sizeof(float));
         r |= dalloc(&c->Apart, (size_t)c->nsplit * c->Mpad * c->kp * sizeof(float));
     }
@@ -757,7 +759,6 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
                                        accumulate ? 1 : 0));
         return 0;
     }
-    uint4 *UWB = c->UWB + (r0 / 4) * c->kp * 2;
     {
         Stage s(c, T_MM_A);
         DPCA_CHECK(launch_za_mx(c->st, blk, rows, c->ldp, c->ztab + r0, c->QB, c->kp,
@@ -765,15 +766,20 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     }
     {
         Stage s(c, T_MM);
-        DPCA_CHECK(launch_tprep(c->st, c->Tpart, c->nseg, rows, c->kp, c->mu + r0, c->dsc + r0,
-                                T, UWB));
+        DPCA_CHECK(launch_tprep(c->st, c->Tpart, c->nseg, rows, c->kp, T));
     }
     if (half) return 0;
     const int nsplit = (int)((rows + c->rps - 1) / c->rps);
+    uint4 *UW8 = c->UW8 + (r0 / 8) * 3 * c->kp;
+    {
+        Stage s(c, T_MM);
+        DPCA_CHECK(launch_uw8(c->st, T, rows, c->kp, c->mu + r0, c->dsc + r0, c->rps, c->smax,
+                              UW8));
+    }
     {
         Stage s(c, T_MM_AT);
-        DPCA_CHECK(launch_zt_mx(c->st, blk, rows, c->ldp, UWB, c->kp, c->sa_tiles, c->rps,
-                                nsplit, c->Apart));
+        DPCA_CHECK(launch_zt_i8(c->st, blk, rows, c->ldp, UW8, c->smax, c->kp, c->sa_tiles,
+                                c->rps, nsplit, c->Apart));
     }
     Stage s(c, T_MM);
     DPCA_CHECK(launch_reduce_f32(c->st, c->Apart, nsplit, c->Mpad * c->kp, out,
@@ -1716,8 +1722,14 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     r |= dalloc(&m->mu, n1);
     r |= dalloc(&m->dsc, n1);
     r |= dalloc(&m->W, n1 * c->kp);
-    if (m->precision == DPCA_MIXED) r |= dalloc(&m->UWB, (size_t)c->Npad / 4 * c->kp * 2 * sizeof(uint4));
-    m->bytes = 6 * n1 + n1 * c->kp + (m->UWB ? (size_t)c->Npad / 4 * c->kp * 32 : 0);
+    const int64_t prps = proj_rps_mx(c->Npad);
+    const size_t uw8_bytes = (size_t)c->Npad / 8 * 3 * c->kp * sizeof(uint4);
+    const size_t smax_bytes = (size_t)((c->Npad + prps - 1) / prps) * c->kp * sizeof(unsigned);
+    if (m->precision == DPCA_MIXED) {
+        r |= dalloc(&m->UW8, uw8_bytes);
+        r |= dalloc(&m->smax, smax_bytes);
+    }
+    m->bytes = 6 * n1 + n1 * c->kp + (m->UW8 ? uw8_bytes + smax_bytes : 0);
     double *inv = nullptr;
     r |= dalloc(&inv, 64 * sizeof(double));
     if (r) {
@@ -1750,7 +1762,8 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->zval, c->zval, 4 * n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->mu, c->mu, n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->dsc, c->dsc, n1, hipMemcpyDeviceToDevice, c->st);
-    if (!r && e == hipSuccess && m->UWB) e = launch_uwb(c->st, m->W, c->Npad, c->kp, m->mu, m->dsc, m->UWB);
+    if (!r && e == hipSuccess && m->UW8)
+        e = launch_uw8(c->st, m->W, c->Npad, c->kp, m->mu, m->dsc, prps, m->smax, m->UW8);
     if (!r && e == hipSuccess) e = hipStreamSynchronize(c->st);
     collect_timers(c);
     (void)hipFree(inv);

@@ -67,19 +67,27 @@ hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, in
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
                                double *out, int accumulate);
 
-// Mixed form (bf16 hi+lo operands, FP32 accumulate); see k_mixed.hip.
+// Mixed form (Z Q: bf16 hi+lo operands, FP32 accumulate; Z^T T: int8
+// codes x int8 slices, int32 accumulate); see k_mixed.hip.
 hipError_t launch_ztab(hipStream_t st, const double *zval, int64_t rows, uint4 *ztab);
 hipError_t launch_qsplit(hipStream_t st, const double *Q, int64_t Mpad, int kp, uint4 *QB);
 hipError_t launch_za_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                         const uint4 *ztab, const uint4 *QB, int kp, int st_tiles,
                         int64_t seg_bytes, int nseg, float *Tpart);
 hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t rows, int kp,
-                        const double *mu, const double *dsc, double *T, uint4 *UWB);
-hipError_t launch_zt_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const uint4 *UWB, int kp, int sa_tiles, int64_t rows_per_split,
-                        int nsplit, float *part);
+                        double *T);
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
                              double *out, int accumulate);
+// Z^T T on int8 MFMA: U = d T, W = -d mu T as three int8 slices per (SNP
+// split of rps rows, column) with a power-of-two scale from smax.
+// UW8: rows/8 x 3 x kp 16-byte words; smax: ceil(rows/rps) x kp.
+hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
+                      const double *dsc, int64_t rps, unsigned *smax, void *UW8);
+hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                        const void *UW8, const unsigned *smax, int kp, int sa_tiles,
+                        int64_t rows_per_split, int nsplit, float *part);
+// sample tiles per wave of k_zt_i8 for kp columns and a batch of ldb bytes
+inline int zt_i8_tiles(int kp, int64_t ldb) { return (kp <= 48 && ldb % 128 == 0) ? 4 : 2; }
 
 // Tall-skinny helpers, FP64.  X, Y: m x kp row-major.
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
@@ -102,8 +110,6 @@ hipError_t launch_fill_cols(hipStream_t st, uint8_t *dst, int64_t rows, int64_t 
                             int64_t c1);
 hipError_t launch_loadings(hipStream_t st, const double *T, int64_t rows, int kp, int nsv,
                            const double *inv_s2, double *W);
-hipError_t launch_uwb(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
-                      const double *dsc, uint4 *UWB);
 
 // Per-iteration state of the convergence test (device resident).
 struct IterState {

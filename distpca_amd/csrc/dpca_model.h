@@ -14,7 +14,8 @@ struct dpca_model {
     double *zval = nullptr;    // Npad x 4, training table (FP64 form)
     double *mu = nullptr, *dsc = nullptr;
     double *W = nullptr;       // Npad x kp loadings Z U diag(1/sigma^2)
-    uint4 *UWB = nullptr;      // mixed form: bf16 hi/lo of d W and -d mu W
+    uint4 *UW8 = nullptr;      // mixed form: int8 slices of d W and -d mu W
+    unsigned *smax = nullptr;  // mixed form: their per (split, column) maxima
     size_t bytes = 0;          // device footprint of the above
 };
 
@@ -24,11 +25,13 @@ namespace dpca {
 // bytes, 4 samples per byte, padding codes 01): scores[(4*ldb) x kp] in FP64,
 // scores[i][j] = sum_s z_si W_sj with z standardised by the training table.
 // `part` holds the split partials (FP64 form: splits x 4*ldb x kp doubles;
-// mixed: nsplit x 4*ldb x kp floats).  Launch parameters are fixed per
+// mixed: nsplit x 4*ldb x kp floats; the int8 slices of the mixed form are
+// scaled per split, so the split is also the model's slice-scale group).  Launch parameters are fixed per
 // (model, ldb) so the launches can be captured in a graph; the SNP split is
 // fixed per model (batch-invariant reduction order).
 constexpr int64_t kProjSplitRowsF64 = 4096;  // SNPs per split, FP64 form
 constexpr int64_t kProjSplitRowsMx = 2048;   // SNPs per split, mixed form
+inline int64_t proj_rps_mx(int64_t Npad) { return Npad < kProjSplitRowsMx ? Npad : kProjSplitRowsMx; }
 struct ProjPlan {
     int64_t ldb = 0;
     int splits = 1;            // FP64 form

@@ -1,22 +1,23 @@
-// Mixed-precision form of one pass  A A^T Q = Z^T (Z Q)  on bf16 MFMA with
-// FP32 accumulation (v_mfma_f32_16x16x32_bf16).
+// Mixed-precision form of one pass  A A^T Q = Z^T (Z Q).
 //
-//   T = Z Q      A operand: Z = hi + lo, each bf16, looked up from a per-SNP
+//   T = Z Q      bf16 MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulation.
+//                A operand: Z = hi + lo, each bf16, looked up from a per-SNP
 //                4-entry table (zval split); B operand: Q = hi + lo, bf16.
 //                Three products per tile (hi*hi + hi*lo + lo*hi), FP32
 //                accumulation over one sample segment; segments summed in
 //                FP64.
-//   Z^T T        Z_si = d_s (h_si - mu_s m_si) with h the genotype count
-//                (0 when missing) and m the non-missing indicator, so
-//                Z^T T = sum_s [h_si U_s + m_si W_s],  U = d T, W = -d mu T.
-//                The raw codes enter the MFMA unscaled: each SNP takes two
-//                K slots (h and m, both exact in bf16) against the bf16 hi+lo
-//                split of U and W, two products per tile.  SNP splits are
-//                summed in FP64.
+//   Z^T T        int8 MFMA (v_mfma_i32_16x16x64_i8), exact int32
+//                accumulation.  Z_si = d_s (h_si - mu_s m_si) with h the
+//                genotype count (0 when missing) and m the non-missing
+//                indicator, so Z^T T = sum_s [h_si U_s + m_si W_s] with
+//                U = d T, W = -d mu T.  The raw codes enter the MFMA unscaled
+//                as two exact int8 K slots per SNP (h and m) against three
+//                int8 fixed-point slices of U and W (one power-of-two scale
+//                per SNP split and column).  SNP splits are summed in FP64.
 //
-// MFMA operand layout (16x16x32): lane l = 16g + r holds A[r][8g..8g+7] and
-// B[8g..8g+7][r] and receives D[4g + q][r].  Which sample (or SNP) occupies
-// which K slot only has to agree between A and B.
+// bf16 MFMA operand layout (16x16x32): lane l = 16g + r holds A[r][8g..8g+7]
+// and B[8g..8g+7][r] and receives D[4g + q][r].  Which sample (or SNP)
+// occupies which K slot only has to agree between A and B.
 #include "dpca_internal.h"
 #include "k_common.h"
 
@@ -230,223 +231,24 @@ hipError_t launch_za_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int
     return hipErrorInvalidValue;
 }
 
-// ------------------------------------------------ T segments -> T, U, W -----
-// T[s][c] = sum of the segment partials (FP64); U = d_s T, W = -d_s mu_s T;
-// UWB[((s/4)*2 + 0)*kp + c] = bf16 hi of U(s0..s3), W(s0..s3);
-// UWB[((s/4)*2 + 1)*kp + c] = the lo parts.
+// ----------------------------------------------------- T segments -> T -----
+// T[s][c] = sum of the sample-segment partials of Z Q, in FP64.
 __global__ void k_tprep(const float *__restrict__ Tpart, int nseg, int64_t rows, int kp,
-                        const double *__restrict__ mu, const double *__restrict__ dsc,
-                        double *__restrict__ T, uint4 *__restrict__ UWB)
+                        double *__restrict__ T)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows / 4 * kp) return;
-    const int64_t s4 = idx / kp;
-    const int c = (int)(idx % kp);
-    uint32_t uh[4], ul[4], wh[4], wl[4];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const int64_t s = s4 * 4 + e;
-        double t = 0;
-        for (int p = 0; p < nseg; p++) t += (double)Tpart[((int64_t)p * rows + s) * kp + c];
-        T[s * kp + c] = t;
-        const double u = dsc[s] * t;
-        const double w = -(dsc[s] * mu[s]) * t;
-        split_bf16(u, uh[e], ul[e]);
-        split_bf16(w, wh[e], wl[e]);
-    }
-    UWB[(s4 * 2) * kp + c] = make_uint4(uh[0] | (uh[1] << 16), uh[2] | (uh[3] << 16),
-                                        wh[0] | (wh[1] << 16), wh[2] | (wh[3] << 16));
-    UWB[(s4 * 2 + 1) * kp + c] = make_uint4(ul[0] | (ul[1] << 16), ul[2] | (ul[3] << 16),
-                                            wl[0] | (wl[1] << 16), wl[2] | (wl[3] << 16));
+    if (idx >= rows * kp) return;
+    double t = 0;
+    for (int p = 0; p < nseg; p++) t += (double)Tpart[(int64_t)p * rows * kp + idx];
+    T[idx] = t;
 }
 
 hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t rows, int kp,
-                        const double *mu, const double *dsc, double *T, uint4 *UWB)
+                        double *T)
 {
-    const int64_t n = rows / 4 * kp;
-    k_tprep<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Tpart, nseg, rows, kp, mu, dsc, T, UWB);
+    const int64_t n = rows * kp;
+    k_tprep<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Tpart, nseg, rows, kp, T);
     return hipGetLastError();
-}
-
-// U/W operand of a resident FP64 right-hand matrix (projection loadings):
-// same layout as k_tprep's output.
-__global__ void k_uwb(const double *__restrict__ T, int64_t rows, int kp,
-                      const double *__restrict__ mu, const double *__restrict__ dsc,
-                      uint4 *__restrict__ UWB)
-{
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows / 4 * kp) return;
-    const int64_t s4 = idx / kp;
-    const int c = (int)(idx % kp);
-    uint32_t uh[4], ul[4], wh[4], wl[4];
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const int64_t s = s4 * 4 + e;
-        const double t = T[s * kp + c];
-        split_bf16(dsc[s] * t, uh[e], ul[e]);
-        split_bf16(-(dsc[s] * mu[s]) * t, wh[e], wl[e]);
-    }
-    UWB[(s4 * 2) * kp + c] = make_uint4(uh[0] | (uh[1] << 16), uh[2] | (uh[3] << 16),
-                                        wh[0] | (wh[1] << 16), wh[2] | (wh[3] << 16));
-    UWB[(s4 * 2 + 1) * kp + c] = make_uint4(ul[0] | (ul[1] << 16), ul[2] | (ul[3] << 16),
-                                            wl[0] | (wl[1] << 16), wl[2] | (wl[3] << 16));
-}
-
-hipError_t launch_uwb(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
-                      const double *dsc, uint4 *UWB)
-{
-    const int64_t n = rows / 4 * kp;
-    k_uwb<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, mu, dsc, UWB);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------ Z^T T part ----
-// Workgroup: 8 waves x SA tiles x 16 samples (SA = 8: 1024 samples = 256
-// packed bytes per row) and one SNP split (blockIdx.y).  128-SNP chunks of
-// packed bytes are staged in LDS transposed to [byte column][SNP], so a lane
-// reads the bytes of 4 consecutive SNPs for its sample with one dword load.
-// K step t (16 SNPs): lane group g holds SNPs 16t + 4g .. +3; K slots 0..3 are
-// their h, slots 4..7 their m.
-template <int SA, int CT>
-__global__ __launch_bounds__(512) void k_zt_mx(const uint8_t *__restrict__ packed, int64_t ldp,
-                                               const uint4 *__restrict__ UWB, int64_t rows,
-                                               int64_t rows_per_split, int64_t Mpad,
-                                               float *__restrict__ part)
-{
-    constexpr int KP = CT * 16;
-    constexpr int TB = SA * 16 * 8 / 4;    // packed bytes per row in this tile
-    constexpr int SC = 128;                // SNPs per chunk
-    // [byte column][4-SNP group], rows padded by one dword so that the
-    // transposing stores (consecutive byte columns) and the reads (4 byte
-    // columns x 4 groups per 16 lanes) spread over the LDS banks
-    __shared__ uint32_t pk[TB][SC / 4 + 1];
-    __shared__ u32x4 uw[(SC / 4) * KP * 2];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int r = lane & 15, g = lane >> 4;
-    const int64_t mb = (int64_t)blockIdx.x * TB;
-    const int64_t sbeg = (int64_t)blockIdx.y * rows_per_split;
-    int64_t send = sbeg + rows_per_split;
-    if (send > rows) send = rows;
-    // h / m tables for perm: bytes indexed by the 2-bit code
-    const uint32_t HL = 0x00800000u, HH = 0x403F0000u, ML = 0x80800080u, MH = 0x3F3F003Fu;
-
-    f4 acc[SA][CT];
-#pragma unroll
-    for (int a = 0; a < SA; a++)
-#pragma unroll
-        for (int c = 0; c < CT; c++) acc[a][c] = f4{0, 0, 0, 0};
-
-    // Software pipeline: the next chunk's packed rows and U/W operands are
-    // loaded into registers while the current chunk is multiplied out of LDS.
-    constexpr int NPK = (SC / 4) * (TB / 4) / 512;   // 4x4 byte blocks per thread
-    constexpr int NUW = (SC / 4) * KP * 2 / 512;     // uint4 of U/W per thread
-    static_assert((SC / 4) * (TB / 4) % 512 == 0 && (SC / 4) * KP * 2 % 512 == 0, "tile");
-    uint32_t rpk[NPK][4];
-    u32x4 ruw[NUW];
-#define DPCA_ZT_LOAD(c0)                                                                    \
-    do {                                                                                    \
-        _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
-            const int it_ = tid + 512 * u;                                                  \
-            const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
-            const uint8_t *p_ = packed + ((c0) + 4 * sg_) * ldp + mb + 4 * bq_;             \
-            _Pragma("unroll") for (int j = 0; j < 4; j++) rpk[u][j] =                      \
-                *reinterpret_cast<const uint32_t *>(p_ + j * ldp);                          \
-        }                                                                                   \
-        const u32x4 *q_ = reinterpret_cast<const u32x4 *>(UWB) + ((c0) / 4) * KP * 2;      \
-        _Pragma("unroll") for (int u = 0; u < NUW; u++) ruw[u] = q_[tid + 512 * u];        \
-    } while (0)
-    DPCA_ZT_LOAD(sbeg < send ? sbeg : 0);
-
-    for (int64_t s0 = sbeg; s0 < send; s0 += SC) {
-        __syncthreads();
-        // 4x4 byte transpose of each block into [byte column][SNP group]
-#pragma unroll
-        for (int u = 0; u < NPK; u++) {
-            const int it = tid + 512 * u;
-            const int sg = it / (TB / 4), bq = it % (TB / 4);
-            const uint32_t r0 = rpk[u][0], r1 = rpk[u][1], r2 = rpk[u][2], r3 = rpk[u][3];
-            const uint32_t lo01 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);
-            const uint32_t hi01 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);
-            const uint32_t lo23 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
-            const uint32_t hi23 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
-            pk[4 * bq + 0][sg] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
-            pk[4 * bq + 1][sg] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
-            pk[4 * bq + 2][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
-            pk[4 * bq + 3][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
-        }
-#pragma unroll
-        for (int u = 0; u < NUW; u++) uw[tid + 512 * u] = ruw[u];
-        __syncthreads();
-        {
-            // unconditional (re-reads the last chunk at the end) so the
-            // prefetch registers stay registers
-            const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
-            DPCA_ZT_LOAD(nx);
-        }
-#pragma unroll 1
-        for (int t = 0; t < SC / 16; t++) {
-            u32x4 bh[CT], bl[CT];
-#pragma unroll
-            for (int c = 0; c < CT; c++) {
-                bh[c] = uw[((4 * t + g) * 2) * KP + c * 16 + r];
-                bl[c] = uw[((4 * t + g) * 2 + 1) * KP + c * 16 + r];
-            }
-#pragma unroll
-            for (int a = 0; a < SA; a++) {
-                const int i = wave * SA * 16 + a * 16 + r;  // sample within the tile
-                const uint32_t wv = pk[i >> 2][4 * t + g];
-                const uint32_t cw = (wv >> (2 * (i & 3))) & 0x03030303u;
-                const uint32_t s01 = __builtin_amdgcn_perm(cw, cw, 0x01010000u) + 0x04000400u;
-                const uint32_t s23 = __builtin_amdgcn_perm(cw, cw, 0x03030202u) + 0x04000400u;
-                const u32x4 av = u32x4{__builtin_amdgcn_perm(HH, HL, s01),
-                                       __builtin_amdgcn_perm(HH, HL, s23),
-                                       __builtin_amdgcn_perm(MH, ML, s01),
-                                       __builtin_amdgcn_perm(MH, ML, s23)};
-#pragma unroll
-                for (int c = 0; c < CT; c++) {
-                    acc[a][c] = mfma_bf16(av, bh[c], acc[a][c]);
-                    acc[a][c] = mfma_bf16(av, bl[c], acc[a][c]);
-                }
-            }
-        }
-    }
-    float *out = part + ((int64_t)blockIdx.y * Mpad + 4 * mb + wave * SA * 16) * KP;
-#pragma unroll
-    for (int a = 0; a < SA; a++)
-#pragma unroll
-        for (int c = 0; c < CT; c++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) out[(a * 16 + 4 * g + q) * KP + c * 16 + r] = acc[a][c][q];
-}
-
-template <int SA>
-static hipError_t zt_mx_ct(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
-                           const uint4 *UWB, int kp, int64_t rows, int64_t rps, int64_t Mpad,
-                           float *part)
-{
-    switch (kp) {
-    case 16: k_zt_mx<SA, 1><<<grid, 512, 0, st>>>(packed, ldp, UWB, rows, rps, Mpad, part); break;
-    case 32: k_zt_mx<SA, 2><<<grid, 512, 0, st>>>(packed, ldp, UWB, rows, rps, Mpad, part); break;
-    case 48: k_zt_mx<SA, 3><<<grid, 512, 0, st>>>(packed, ldp, UWB, rows, rps, Mpad, part); break;
-    case 64: k_zt_mx<SA, 4><<<grid, 512, 0, st>>>(packed, ldp, UWB, rows, rps, Mpad, part); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-// ldp must be a multiple of the tile's byte width (SA*32), rows of 128,
-// rows_per_split a multiple of 128.
-hipError_t launch_zt_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const uint4 *UWB, int kp, int sa_tiles, int64_t rows_per_split,
-                        int nsplit, float *part)
-{
-    const int64_t tb = (int64_t)sa_tiles * 32;
-    if (ldp % tb || rows % 128 || rows_per_split % 128) return hipErrorInvalidValue;
-    dim3 grid((unsigned)(ldp / tb), (unsigned)nsplit);
-    const int64_t Mpad = 4 * ldp;
-    if (sa_tiles == 8) return zt_mx_ct<8>(st, grid, packed, ldp, UWB, kp, rows, rows_per_split, Mpad, part);
-    if (sa_tiles == 2) return zt_mx_ct<2>(st, grid, packed, ldp, UWB, kp, rows, rows_per_split, Mpad, part);
-    return hipErrorInvalidValue;
 }
 
 // out[e] (+)= sum over splits of float partials, in FP64, split order.
@@ -465,6 +267,292 @@ hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int6
 {
     k_reduce_f32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, n, out, accumulate);
     return hipGetLastError();
+}
+
+// ====================================================== Z^T T on int8 MFMA ==
+// Z^T T = sum_s [h_si U_s + m_si W_s] again, now on v_mfma_i32_16x16x64_i8:
+// the codes give h (0/1/2) and m (0/1) as exact int8 K slots, and U, W are
+// split into three int8 slices of a fixed-point form with one power-of-two
+// scale per (SNP split, column):  U = s (q0 + q1 2^-7 + q2 2^-14) + e with
+// |e| <= s 2^-15 and s >= max(|U|, |W|) / 127 over the split.  Within a split
+// the slice products accumulate exactly in int32 (|sum| <= 381 per SNP), and
+// the three slice sums are combined in FP64 at the end of the split.
+//
+// MFMA operand layout (16x16x64, checked on the device): lane l = 16g + r
+// holds A[r][16g..16g+15] and B[16g..16g+15][r] as bytes and receives
+// D[4g + q][r].  K step t covers 32 SNPs; lane group g takes SNPs
+// 32t + 8g .. +7 with K bytes [h(0..3), h(4..7), m(0..3), m(4..7)].
+//
+// UW8 layout: per 8-SNP group G, per slice j, per column c one 16-byte word
+// {U_j(8G..8G+3), U_j(8G+4..+7), W_j(8G..+3), W_j(8G+4..+7)} at
+// UW8[(G * 3 + j) * kp + c]; smax[split * kp + c] is the column's max of
+// |U|, |W| over the split, rounded up to float.
+
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ i32x4 mfma_i8(const u32x4 &a, const u32x4 &b, const i32x4 &c)
+{
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(__builtin_bit_cast(i32x4, a),
+                                                 __builtin_bit_cast(i32x4, b), c, 0, 0, 0);
+}
+
+// power-of-two scale s with max / s < 127 (s = 1 for an all-zero column)
+__device__ __forceinline__ double slice_scale(float mx)
+{
+    int e = 0;
+    (void)frexp((double)mx / 127.0, &e);
+    return ldexp(1.0, e);
+}
+
+// |x| <= 127 s: three int8 slices of x / s, each residual scaled by 2^7.
+__device__ __forceinline__ void slices3(double x, double inv_s, int &q0, int &q1, int &q2)
+{
+    double v = x * inv_s;              // exact: inv_s is a power of two
+    const double a = rint(v);
+    v = (v - a) * 128.0;               // exact, |v| <= 64
+    const double b = rint(v);
+    v = (v - b) * 128.0;
+    q0 = (int)a;
+    q1 = (int)b;
+    q2 = (int)rint(v);
+}
+
+// smax[split * kp + c] = max over rows of the split of f_s |T_sc|, with
+// f_s = |d_s| max(1, |mu_s|) (covers |U| = |d T| and |W| = |d mu T|).
+// Block: 4 x kp threads over 1024 rows of one split; one atomic per column.
+// smax must hold zeros (or smaller maxima) on entry.
+__global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
+                            const double *__restrict__ mu, const double *__restrict__ dsc,
+                            int64_t rps, unsigned *__restrict__ smax)
+{
+    __shared__ float red[256];
+    const int tid = threadIdx.x, rl = tid / kp, c = tid % kp;
+    const int64_t split = blockIdx.y;
+    const int64_t r0 = split * rps + (int64_t)blockIdx.x * 1024;
+    int64_t r1 = r0 + 1024;
+    if (r1 > (split + 1) * rps) r1 = (split + 1) * rps;
+    if (r1 > rows) r1 = rows;
+    double m = 0;
+    for (int64_t s = r0 + rl; s < r1; s += 4) {
+        const double f = fabs(dsc[s]) * fmax(1.0, fabs(mu[s]));
+        m = fmax(m, f * fabs(T[s * kp + c]));
+    }
+    float f = (float)m;
+    if ((double)f < m) f = __uint_as_float(__float_as_uint(f) + 1u);  // round up
+    red[tid] = f;
+    __syncthreads();
+    if (rl == 0) {
+        for (int j = 1; j < 4; j++) f = fmaxf(f, red[j * kp + c]);
+        if (f > 0) atomicMax(&smax[split * kp + c], __float_as_uint(f));
+    }
+}
+
+// UW8 for rows [0, rows) of T (U = d T, W = -d mu T), split rps rows.
+__global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp,
+                      const double *__restrict__ mu, const double *__restrict__ dsc, int64_t rps,
+                      const unsigned *__restrict__ smax, u32x4 *__restrict__ UW8)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows / 8 * kp) return;
+    const int64_t G = idx / kp;
+    const int c = (int)(idx % kp);
+    const int64_t split = (G * 8) / rps;
+    const double s = slice_scale(__uint_as_float(smax[split * kp + c]));
+    const double inv = 1.0 / s;
+    uint32_t w[3][4] = {};
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int64_t row = G * 8 + e;
+        const double t = T[row * kp + c];
+        int u0, u1, u2, v0, v1, v2;
+        slices3(dsc[row] * t, inv, u0, u1, u2);
+        slices3(-(dsc[row] * mu[row]) * t, inv, v0, v1, v2);
+        const int sh = 8 * (e & 3), wu = e >> 2, ww = 2 + (e >> 2);
+        w[0][wu] |= (uint32_t)(u0 & 0xFF) << sh;
+        w[1][wu] |= (uint32_t)(u1 & 0xFF) << sh;
+        w[2][wu] |= (uint32_t)(u2 & 0xFF) << sh;
+        w[0][ww] |= (uint32_t)(v0 & 0xFF) << sh;
+        w[1][ww] |= (uint32_t)(v1 & 0xFF) << sh;
+        w[2][ww] |= (uint32_t)(v2 & 0xFF) << sh;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) UW8[(G * 3 + j) * kp + c] = u32x4{w[j][0], w[j][1], w[j][2], w[j][3]};
+}
+
+hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
+                      const double *dsc, int64_t rps, unsigned *smax, void *UW8)
+{
+    if (rows % 128 || rps % 128 || rps < 128 || kp * 4 > 256) return hipErrorInvalidValue;
+    const int64_t nsplit = (rows + rps - 1) / rps;
+    // a fill kernel, not a memset node: safe inside a captured graph
+    hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(smax), nsplit * kp * 4, 0);
+    if (e != hipSuccess) return e;
+    dim3 g1((unsigned)((rps + 1023) / 1024), (unsigned)nsplit);
+    k_split_max<<<g1, 4 * kp, 0, st>>>(T, rows, kp, mu, dsc, rps, smax);
+    const int64_t n = rows / 8 * kp;
+    k_uw8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, mu, dsc, rps, smax,
+                                                        reinterpret_cast<u32x4 *>(UW8));
+    return hipGetLastError();
+}
+
+// Workgroup: 8 waves x SA tiles x 16 samples (SA = 4: 512 samples = 128
+// packed bytes per row) and one SNP split (blockIdx.y); 128-SNP chunks of
+// packed bytes staged in LDS transposed to [byte column][4-SNP group] as in
+// k_zt_mx, and the chunk's UW8 words copied to LDS.
+template <int SA, int CT>
+__global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packed, int64_t ldp,
+                                               const u32x4 *__restrict__ UW8,
+                                               const unsigned *__restrict__ smax, int64_t rows,
+                                               int64_t rows_per_split, int64_t Mpad,
+                                               float *__restrict__ part)
+{
+    constexpr int KP = CT * 16;
+    constexpr int NS = 3;                  // int8 slices
+    constexpr int TB = SA * 16 * 8 / 4;    // packed bytes per row in this tile
+    constexpr int SC = 128;                // SNPs per chunk
+    constexpr int NUWT = (SC / 8) * NS * KP;  // UW8 words per chunk
+    __shared__ uint32_t pk[TB][SC / 4 + 1];
+    __shared__ u32x4 uw[NUWT];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int64_t mb = (int64_t)blockIdx.x * TB;
+    const int64_t sbeg = (int64_t)blockIdx.y * rows_per_split;
+    int64_t send = sbeg + rows_per_split;
+    if (send > rows) send = rows;
+    const uint32_t HT = 0x02010000u, MT = 0x01010001u;  // h, m by 2-bit code
+
+    i32x4 acc[SA][CT][NS];
+#pragma unroll
+    for (int a = 0; a < SA; a++)
+#pragma unroll
+        for (int c = 0; c < CT; c++)
+#pragma unroll
+            for (int j = 0; j < NS; j++) acc[a][c][j] = i32x4{0, 0, 0, 0};
+
+    constexpr int NPK = (SC / 4) * (TB / 4) / 512;   // 4x4 byte blocks per thread
+    constexpr int NUW = (NUWT + 511) / 512;          // UW8 words per thread
+    static_assert((SC / 4) * (TB / 4) % 512 == 0, "tile");
+    uint32_t rpk[NPK][4];
+    u32x4 ruw[NUW];
+#define DPCA_ZT8_LOAD(c0)                                                                   \
+    do {                                                                                    \
+        _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
+            const int it_ = tid + 512 * u;                                                  \
+            const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
+            const uint8_t *p_ = packed + ((c0) + 4 * sg_) * ldp + mb + 4 * bq_;             \
+            _Pragma("unroll") for (int j = 0; j < 4; j++) rpk[u][j] =                      \
+                *reinterpret_cast<const uint32_t *>(p_ + j * ldp);                          \
+        }                                                                                   \
+        const u32x4 *q_ = UW8 + ((c0) / 8) * NS * KP;                                       \
+        _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
+            const int it_ = tid + 512 * u;                                                  \
+            if (it_ < NUWT) ruw[u] = q_[it_];                                               \
+        }                                                                                   \
+    } while (0)
+    DPCA_ZT8_LOAD(sbeg < send ? sbeg : 0);
+
+    for (int64_t s0 = sbeg; s0 < send; s0 += SC) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NPK; u++) {
+            const int it = tid + 512 * u;
+            const int sg = it / (TB / 4), bq = it % (TB / 4);
+            const uint32_t r0 = rpk[u][0], r1 = rpk[u][1], r2 = rpk[u][2], r3 = rpk[u][3];
+            const uint32_t lo01 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);
+            const uint32_t hi01 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);
+            const uint32_t lo23 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
+            const uint32_t hi23 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
+            pk[4 * bq + 0][sg] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
+            pk[4 * bq + 1][sg] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
+            pk[4 * bq + 2][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
+            pk[4 * bq + 3][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
+        }
+#pragma unroll
+        for (int u = 0; u < NUW; u++) {
+            const int it = tid + 512 * u;
+            if (it < NUWT) uw[it] = ruw[u];
+        }
+        __syncthreads();
+        {
+            const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
+            DPCA_ZT8_LOAD(nx);
+        }
+#pragma unroll 1
+        for (int t = 0; t < SC / 32; t++) {
+            u32x4 av[SA];
+#pragma unroll
+            for (int a = 0; a < SA; a++) {
+                const int i = wave * SA * 16 + a * 16 + r;  // sample within the tile
+                const int sh = 2 * (i & 3);
+                const uint32_t c0 = (pk[i >> 2][8 * t + 2 * g] >> sh) & 0x03030303u;
+                const uint32_t c1 = (pk[i >> 2][8 * t + 2 * g + 1] >> sh) & 0x03030303u;
+                av[a] = u32x4{__builtin_amdgcn_perm(0u, HT, c0), __builtin_amdgcn_perm(0u, HT, c1),
+                              __builtin_amdgcn_perm(0u, MT, c0), __builtin_amdgcn_perm(0u, MT, c1)};
+            }
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                u32x4 b[CT];
+#pragma unroll
+                for (int c = 0; c < CT; c++) b[c] = uw[((4 * t + g) * NS + j) * KP + c * 16 + r];
+#pragma unroll
+                for (int a = 0; a < SA; a++)
+#pragma unroll
+                    for (int c = 0; c < CT; c++) acc[a][c][j] = mfma_i8(av[a], b[c], acc[a][c][j]);
+            }
+        }
+    }
+#undef DPCA_ZT8_LOAD
+    float *out = part + ((int64_t)blockIdx.y * Mpad + 4 * mb + wave * SA * 16) * KP;
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const double s = slice_scale(__uint_as_float(smax[blockIdx.y * KP + c * 16 + r]));
+#pragma unroll
+        for (int a = 0; a < SA; a++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double v = (double)acc[a][c][0][q] + ldexp((double)acc[a][c][1][q], -7) +
+                                 ldexp((double)acc[a][c][2][q], -14);
+                out[(a * 16 + 4 * g + q) * KP + c * 16 + r] = (float)(s * v);
+            }
+    }
+}
+
+template <int SA>
+static hipError_t zt_i8_ct(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
+                           const u32x4 *UW8, const unsigned *smax, int kp, int64_t rows,
+                           int64_t rps, int64_t Mpad, float *part)
+{
+    switch (kp) {
+    case 16: k_zt_i8<SA, 1><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part); break;
+    case 32: k_zt_i8<SA, 2><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part); break;
+    case 48: k_zt_i8<SA, 3><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part); break;
+    case 64:
+        // 4 tiles x 4 column tiles x 3 slices would not fit the registers
+        if constexpr (SA <= 2) {
+            k_zt_i8<SA, 4><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part);
+            break;
+        }
+        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ldp must be a multiple of the tile's byte width (SA*32), rows and
+// rows_per_split multiples of 128, rows_per_split at most 2^22 (int32 sums).
+hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                        const void *UW8, const unsigned *smax, int kp, int sa_tiles,
+                        int64_t rows_per_split, int nsplit, float *part)
+{
+    const int64_t tb = (int64_t)sa_tiles * 32;
+    if (ldp % tb || rows % 128 || rows_per_split % 128 || rows_per_split > (1 << 22))
+        return hipErrorInvalidValue;
+    dim3 grid((unsigned)(ldp / tb), (unsigned)nsplit);
+    const int64_t Mpad = 4 * ldp;
+    const u32x4 *uw = reinterpret_cast<const u32x4 *>(UW8);
+    if (sa_tiles == 4) return zt_i8_ct<4>(st, grid, packed, ldp, uw, smax, kp, rows, rows_per_split, Mpad, part);
+    if (sa_tiles == 2) return zt_i8_ct<2>(st, grid, packed, ldp, uw, smax, kp, rows, rows_per_split, Mpad, part);
+    return hipErrorInvalidValue;
 }
 
 }  // namespace dpca
