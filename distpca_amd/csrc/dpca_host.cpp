@@ -162,8 +162,9 @@ struct dpca_ctx {
            *dots = nullptr;
     int ngparts = 256;
     // mixed form
-    uint4 *ztab = nullptr, *QB = nullptr, *UW8 = nullptr;
+    uint4 *UW8 = nullptr, *Q8 = nullptr;
     unsigned *smax = nullptr;         // per (SNP split, column) max |U|, |W|
+    unsigned *qmax = nullptr;         // per (sample segment, column) max |Q|
     float *Tpart = nullptr, *Apart = nullptr;
     int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 4;
     int64_t seg_bytes = 0, rps = 0;
@@ -262,7 +263,7 @@ static void free_ctx(dpca_ctx *c)
     (void)hipSetDevice(c->dev);
     void *ptrs[] = {c->packed, c->zval, c->mu, c->dsc, c->ssum, c->sctr, c->Q, c->ARHS,
                     c->RHS2, c->RHS2old, c->tmp, c->T, c->part, c->gpart, c->G, c->Rinv,
-                    c->w, c->X, c->dots, c->flags, c->its, c->ztab, c->QB, c->UW8, c->smax, c->Tpart, c->Apart};
+                    c->w, c->X, c->dots, c->flags, c->its, c->UW8, c->Q8, c->qmax, c->smax, c->Tpart, c->Apart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int b = 0; b < 2; b++) {
@@ -380,8 +381,8 @@ static int alloc_work(dpca_ctx *c)
         const int64_t nsegs = (c->ldp + 4095) / 4096;
         c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 64);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
-        c->st_tiles = (B / 512 * c->nseg >= 512) ? 4 : 2;
-        if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 2, 4, 8
+        c->st_tiles = (c->kp <= 48) ? 2 : 1;
+        if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 1, 2
         // Z^T T: SNP splits of <= 16384 rows, >= ~1024 workgroups
         c->sa_tiles = zt_i8_tiles(c->kp, c->ldp);
         if (const char *e = getenv("DPCA_ZT_TILES")) c->sa_tiles = atoi(e);  // tuning knob: 2, 4
@@ -390,8 +391,8 @@ static int alloc_work(dpca_ctx *c)
         ns = std::min<int64_t>(ns, B / 128);
         c->rps = round_up((B + ns - 1) / ns, 128);
         c->nsplit = (int)((B + c->rps - 1) / c->rps);
-        r |= dalloc(&c->ztab, (size_t)c->Npad * sizeof(uint4));
-        r |= dalloc(&c->QB, (size_t)c->Mpad / 8 * c->kp * 2 * sizeof(uint4));
+        r |= dalloc(&c->Q8, (size_t)c->Mpad / 16 * 3 * c->kp * sizeof(uint4));
+        r |= dalloc(&c->qmax, (size_t)c->nseg * c->kp * sizeof(unsigned));
         r |= dalloc(&c->UW8, (size_t)c->Npad / 8 * 3 * c->kp * sizeof(uint4));
         r |= dalloc(&c->smax, (size_t)c->nsplit * c->kp * sizeof(unsigned));
         r |= dalloc(&c->Tpart, (size_t)c->nseg * B * c->kp * sizeof(float));
@@ -423,8 +424,6 @@ static int block_stats(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows
     DPCA_CHECK(launch_snp_stats(c->st, blk, rows, c->ldp, 1.0 / sqrt((double)c->N),
                                 c->ssum + r0, c->sctr + r0, c->zval + 4 * r0, c->mu + r0,
                                 c->dsc + r0));
-    if (c->p.precision == DPCA_MIXED)
-        DPCA_CHECK(launch_ztab(c->st, c->zval + 4 * r0, rows, c->ztab + r0));
     return 0;
 }
 
@@ -730,12 +729,12 @@ static int upload_rows(dpca_ctx *c, const uint8_t *host, int64_t rows)
 // streamed from pinned host memory.  Each block adds its Z_b^T (Z_b Qin)
 // into `out` (FP64, block order); `half` stops after T = Z_b Qin.
 
-// Mixed form: Q -> bf16 hi/lo once per pass.
+// Mixed form: Q -> int8 slices once per pass.
 static int pass_prologue(dpca_ctx *c, const double *Qin)
 {
     if (c->p.precision != DPCA_MIXED) return 0;
     Stage s(c, T_MM);
-    DPCA_CHECK(launch_qsplit(c->st, Qin, c->Mpad, c->kp, c->QB));
+    DPCA_CHECK(launch_q8(c->st, Qin, c->Mpad, c->kp, c->seg_bytes, c->qmax, c->Q8));
     return 0;
 }
 
@@ -761,8 +760,9 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     }
     {
         Stage s(c, T_MM_A);
-        DPCA_CHECK(launch_za_mx(c->st, blk, rows, c->ldp, c->ztab + r0, c->QB, c->kp,
-                                c->st_tiles, c->seg_bytes, c->nseg, c->Tpart));
+        DPCA_CHECK(launch_za_i8(c->st, blk, rows, c->ldp, c->Q8, c->qmax, c->mu + r0,
+                                c->dsc + r0, c->kp, c->st_tiles, c->seg_bytes, c->nseg,
+                                c->Tpart));
     }
     {
         Stage s(c, T_MM);

@@ -67,13 +67,8 @@ hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, in
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
                                double *out, int accumulate);
 
-// Mixed form (Z Q: bf16 hi+lo operands, FP32 accumulate; Z^T T: int8
-// codes x int8 slices, int32 accumulate); see k_mixed.hip.
-hipError_t launch_ztab(hipStream_t st, const double *zval, int64_t rows, uint4 *ztab);
-hipError_t launch_qsplit(hipStream_t st, const double *Q, int64_t Mpad, int kp, uint4 *QB);
-hipError_t launch_za_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const uint4 *ztab, const uint4 *QB, int kp, int st_tiles,
-                        int64_t seg_bytes, int nseg, float *Tpart);
+// Mixed form: exact int8 codes x three int8 slices of the FP64 operand,
+// int32 accumulation; see k_mixed.hip.
 hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t rows, int kp,
                         double *T);
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
@@ -86,6 +81,15 @@ hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int kp, con
 hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                         const void *UW8, const unsigned *smax, int kp, int sa_tiles,
                         int64_t rows_per_split, int nsplit, float *part);
+// Z Q on int8 MFMA: codes as exact h / m rows, Q as three int8 slices per
+// (sample segment of 4*seg_bytes samples, column) with a power-of-two scale
+// from qmax.  Q8: Mpad/16 x 3 x kp 16-byte words; qmax: nseg x kp.
+hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int kp, int64_t seg_bytes,
+                     unsigned *qmax, void *Q8);
+hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                        const void *Q8, const unsigned *qmax, const double *mu,
+                        const double *dsc, int kp, int st_tiles, int64_t seg_bytes, int nseg,
+                        float *Tpart);
 // sample tiles per wave of k_zt_i8 for kp columns and a batch of ldb bytes
 inline int zt_i8_tiles(int kp, int64_t ldb) { return (kp <= 48 && ldb % 128 == 0) ? 4 : 2; }
 

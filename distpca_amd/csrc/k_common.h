@@ -7,8 +7,6 @@ namespace dpca {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
-typedef short s8v __attribute__((ext_vector_type(8)));
-typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
 // 64-wide wavefront sum.
 __device__ __forceinline__ double wave_sum(double v)

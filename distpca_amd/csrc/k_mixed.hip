@@ -1,234 +1,34 @@
-// Mixed-precision form of one pass  A A^T Q = Z^T (Z Q).
+// Mixed-precision form of one pass  A A^T Q = Z^T (Z Q)  on int8 MFMA
+// (v_mfma_i32_16x16x64_i8).  The genotype codes enter the products as exact
+// int8 values -- h, the allele count (0 when missing), and m, the
+// non-missing indicator -- with Z_si = d_s (h_si - mu_s m_si); the FP64
+// right-hand operand is split into three int8 fixed-point slices with one
+// power-of-two scale per column and group of rows, so every product sums
+// exactly in int32 and only the final combination rounds.
 //
-//   T = Z Q      bf16 MFMA (v_mfma_f32_16x16x32_bf16), FP32 accumulation.
-//                A operand: Z = hi + lo, each bf16, looked up from a per-SNP
-//                4-entry table (zval split); B operand: Q = hi + lo, bf16.
-//                Three products per tile (hi*hi + hi*lo + lo*hi), FP32
-//                accumulation over one sample segment; segments summed in
-//                FP64.
-//   Z^T T        int8 MFMA (v_mfma_i32_16x16x64_i8), exact int32
-//                accumulation.  Z_si = d_s (h_si - mu_s m_si) with h the
-//                genotype count (0 when missing) and m the non-missing
-//                indicator, so Z^T T = sum_s [h_si U_s + m_si W_s] with
-//                U = d T, W = -d mu T.  The raw codes enter the MFMA unscaled
-//                as two exact int8 K slots per SNP (h and m) against three
-//                int8 fixed-point slices of U and W (one power-of-two scale
-//                per SNP split and column).  SNP splits are summed in FP64.
+//   T = Z Q      two int8 rows per SNP (h and m) against three slices of Q
+//                (scale per sample segment); T_s = d_s (H_s - mu_s M_s) in
+//                FP64 per segment, FP32 partials, segments summed in FP64.
+//   Z^T T        two int8 K slots per SNP (h and m) against three slices of
+//                U = d T and W = -d mu T (scale per SNP split); splits
+//                summed in FP64.
 //
-// bf16 MFMA operand layout (16x16x32): lane l = 16g + r holds A[r][8g..8g+7]
-// and B[8g..8g+7][r] and receives D[4g + q][r].  Which sample (or SNP)
-// occupies which K slot only has to agree between A and B.
+// MFMA operand layout (16x16x64, checked on the device): lane l = 16g + r
+// holds A[r][16g..16g+15] and B[16g..16g+15][r] as bytes and receives
+// D[4g + q][r].  Which sample (or SNP) occupies which K slot only has to
+// agree between A and B.
 #include "dpca_internal.h"
 #include "k_common.h"
 
 namespace dpca {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-__device__ __forceinline__ uint32_t bf16_bits(float f)  // round to nearest even
-{
-    uint32_t u = __float_as_uint(f);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return u >> 16;
-}
-
-// hi = bf16(x), lo = bf16(x - hi) for a double x
-__device__ __forceinline__ void split_bf16(double x, uint32_t &hi, uint32_t &lo)
-{
-    hi = bf16_bits((float)x);
-    const double h = (double)__uint_as_float(hi << 16);
-    lo = bf16_bits((float)(x - h));
-}
-
-__device__ __forceinline__ f4 mfma_bf16(const u32x4 &a, const u32x4 &b, const f4 &c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
-                                                   __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
-}
 
 // Spread the four 2-bit codes of byte b into the four bytes of a dword.
 __device__ __forceinline__ uint32_t spread4(uint32_t b)
 {
     uint32_t v = (b | (b << 12)) & 0x000F000Fu;
     return (v | (v << 6)) & 0x03030303u;
-}
-
-// ----------------------------------------------------------- per-SNP tables --
-// ztab[s] = {hi(z0)|hi(z1)<<16, hi(z2)|hi(z3)<<16, lo(z0)|lo(z1)<<16, lo(z2)|lo(z3)<<16}
-// where z_c = zval[4s + c] is the reference value for 2-bit code c.
-__global__ void k_ztab(const double *__restrict__ zval, int64_t rows, uint4 *__restrict__ ztab)
-{
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= rows) return;
-    uint32_t h[4], l[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) split_bf16(zval[4 * s + c], h[c], l[c]);
-    ztab[s] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), l[0] | (l[1] << 16),
-                         l[2] | (l[3] << 16));
-}
-
-hipError_t launch_ztab(hipStream_t st, const double *zval, int64_t rows, uint4 *ztab)
-{
-    k_ztab<<<(unsigned)((rows + 255) / 256), 256, 0, st>>>(zval, rows, ztab);
-    return hipGetLastError();
-}
-
-// --------------------------------------------------------------- Q split ----
-// QB[((i/8)*2 + 0)*kp + c] = 8 bf16 hi of Q[i..i+7][c], [((i/8)*2 + 1)*kp + c]
-// = the 8 lo parts: hi and lo rows apart, so 16 lanes reading 16 columns
-// touch 256 contiguous bytes of LDS (no bank conflicts).
-__global__ void k_qsplit(const double *__restrict__ Q, int64_t groups, int kp,
-                         uint4 *__restrict__ QB)
-{
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= groups * kp) return;
-    const int64_t g8 = idx / kp;
-    const int c = (int)(idx % kp);
-    uint32_t h[8], l[8];
-#pragma unroll
-    for (int e = 0; e < 8; e++) split_bf16(Q[(g8 * 8 + e) * kp + c], h[e], l[e]);
-    QB[(g8 * 2) * kp + c] = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16),
-                                       h[4] | (h[5] << 16), h[6] | (h[7] << 16));
-    QB[(g8 * 2 + 1) * kp + c] = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16),
-                                           l[4] | (l[5] << 16), l[6] | (l[7] << 16));
-}
-
-hipError_t launch_qsplit(hipStream_t st, const double *Q, int64_t Mpad, int kp, uint4 *QB)
-{
-    const int64_t n = Mpad / 8 * kp;
-    k_qsplit<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Q, Mpad / 8, kp, QB);
-    return hipGetLastError();
-}
-
-// --------------------------------------------------------------- T = Z Q ----
-// Workgroup: 8 waves x ST tiles x 16 SNPs, one sample segment (blockIdx.y).
-// Per 256-sample chunk, lane group g owns the 64 samples of its 16 packed
-// bytes; at K step t it supplies samples 64g + 8t .. 64g + 8t + 7.  The bf16
-// hi/lo pairs of the chunk's Q rows are staged in LDS.
-template <int ST, int CT>
-__global__ __launch_bounds__(512) void k_za_mx(const uint8_t *__restrict__ packed, int64_t ldp,
-                                               const uint4 *__restrict__ ztab,
-                                               const uint4 *__restrict__ QB,
-                                               int64_t seg_bytes, int64_t Npad,
-                                               float *__restrict__ Tpart)
-{
-    constexpr int KP = CT * 16;
-    __shared__ u32x4 qs[32 * KP * 2];  // [group of 8 samples][hi, lo][col]
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int r = lane & 15, g = lane >> 4;
-    const int64_t base = (int64_t)blockIdx.x * (8 * ST * 16) + wave * (ST * 16);
-    const int64_t b0 = (int64_t)blockIdx.y * seg_bytes;
-    int64_t b1 = b0 + seg_bytes;
-    if (b1 > ldp) b1 = ldp;
-
-    u32x4 tab[ST];
-    f4 acc[ST][CT];
-#pragma unroll
-    for (int s = 0; s < ST; s++) {
-        tab[s] = reinterpret_cast<const u32x4 *>(ztab)[base + s * 16 + r];
-#pragma unroll
-        for (int c = 0; c < CT; c++) acc[s][c] = f4{0, 0, 0, 0};
-    }
-
-    // Software pipeline: the next chunk's Q operand and packed bytes are
-    // loaded into registers while the current chunk is multiplied.
-    constexpr int NQ = 32 * KP * 2 / 512;
-    static_assert(32 * KP * 2 % 512 == 0, "tile");
-    u32x4 rq[NQ], rp[ST];
-#define DPCA_ZA_LOAD(c0)                                                                    \
-    do {                                                                                    \
-        const u32x4 *src_ = reinterpret_cast<const u32x4 *>(QB) + ((c0) / 2) * KP * 2;      \
-        _Pragma("unroll") for (int u = 0; u < NQ; u++) rq[u] = src_[tid + 512 * u];        \
-        _Pragma("unroll") for (int s = 0; s < ST; s++) rp[s] =                             \
-            *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + (c0) +     \
-                                             16 * g);                                       \
-    } while (0)
-    DPCA_ZA_LOAD(b0 < b1 ? b0 : 0);
-
-    for (int64_t cb = b0; cb < b1; cb += 64) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NQ; u++) qs[tid + 512 * u] = rq[u];
-        u32x4 pk[ST];
-#pragma unroll
-        for (int s = 0; s < ST; s++) pk[s] = rp[s];
-        __syncthreads();
-        {
-            const int64_t nx = (cb + 64 < b1) ? cb + 64 : cb;
-            DPCA_ZA_LOAD(nx);
-        }
-#pragma unroll 2
-        for (int t = 0; t < 8; t++) {
-            u32x4 bh[CT], bl[CT];
-#pragma unroll
-            for (int c = 0; c < CT; c++) {
-                bh[c] = qs[((8 * g + t) * 2) * KP + c * 16 + r];
-                bl[c] = qs[((8 * g + t) * 2 + 1) * KP + c * 16 + r];
-            }
-#pragma unroll
-            for (int s = 0; s < ST; s++) {
-                const uint32_t w = pk[s][t >> 1];
-                const uint32_t x16 = (t & 1) ? (w >> 16) : (w & 0xFFFFu);
-                const uint32_t c03 = spread4(x16 & 0xFFu), c47 = spread4(x16 >> 8);
-                uint32_t sel[4];
-                sel[0] = (__builtin_amdgcn_perm(c03, c03, 0x01010000u) << 1) + 0x01000100u;
-                sel[1] = (__builtin_amdgcn_perm(c03, c03, 0x03030202u) << 1) + 0x01000100u;
-                sel[2] = (__builtin_amdgcn_perm(c47, c47, 0x01010000u) << 1) + 0x01000100u;
-                sel[3] = (__builtin_amdgcn_perm(c47, c47, 0x03030202u) << 1) + 0x01000100u;
-                u32x4 ah, al;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    ah[j] = __builtin_amdgcn_perm(tab[s][1], tab[s][0], sel[j]);
-                    al[j] = __builtin_amdgcn_perm(tab[s][3], tab[s][2], sel[j]);
-                }
-#pragma unroll
-                for (int c = 0; c < CT; c++) {
-                    acc[s][c] = mfma_bf16(ah, bh[c], acc[s][c]);
-                    acc[s][c] = mfma_bf16(ah, bl[c], acc[s][c]);
-                    acc[s][c] = mfma_bf16(al, bh[c], acc[s][c]);
-                }
-            }
-        }
-    }
-    float *out = Tpart + (int64_t)blockIdx.y * Npad * KP;
-#pragma unroll
-    for (int s = 0; s < ST; s++)
-#pragma unroll
-        for (int c = 0; c < CT; c++)
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-                out[(base + s * 16 + 4 * g + q) * KP + c * 16 + r] = acc[s][c][q];
-}
-
-template <int ST>
-static hipError_t za_mx_ct(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
-                           const uint4 *ztab, const uint4 *QB, int kp, int64_t seg,
-                           int64_t Npad, float *Tpart)
-{
-    switch (kp) {
-    case 16: k_za_mx<ST, 1><<<grid, 512, 0, st>>>(packed, ldp, ztab, QB, seg, Npad, Tpart); break;
-    case 32: k_za_mx<ST, 2><<<grid, 512, 0, st>>>(packed, ldp, ztab, QB, seg, Npad, Tpart); break;
-    case 48: k_za_mx<ST, 3><<<grid, 512, 0, st>>>(packed, ldp, ztab, QB, seg, Npad, Tpart); break;
-    case 64: k_za_mx<ST, 4><<<grid, 512, 0, st>>>(packed, ldp, ztab, QB, seg, Npad, Tpart); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-// rows must be a multiple of 8*16*st_tiles; seg_bytes a multiple of 64.
-hipError_t launch_za_mx(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const uint4 *ztab, const uint4 *QB, int kp, int st_tiles,
-                        int64_t seg_bytes, int nseg, float *Tpart)
-{
-    if (seg_bytes % 64 || ldp % 64) return hipErrorInvalidValue;
-    const int64_t per = 8 * 16 * st_tiles;
-    if (rows % per) return hipErrorInvalidValue;
-    dim3 grid((unsigned)(rows / per), (unsigned)nseg);
-    if (st_tiles == 8) return za_mx_ct<8>(st, grid, packed, ldp, ztab, QB, kp, seg_bytes, rows, Tpart);
-    if (st_tiles == 4) return za_mx_ct<4>(st, grid, packed, ldp, ztab, QB, kp, seg_bytes, rows, Tpart);
-    if (st_tiles == 2) return za_mx_ct<2>(st, grid, packed, ldp, ztab, QB, kp, seg_bytes, rows, Tpart);
-    return hipErrorInvalidValue;
 }
 
 // ----------------------------------------------------- T segments -> T -----
@@ -270,7 +70,7 @@ hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int6
 }
 
 // ====================================================== Z^T T on int8 MFMA ==
-// Z^T T = sum_s [h_si U_s + m_si W_s] again, now on v_mfma_i32_16x16x64_i8:
+// Z^T T = sum_s [h_si U_s + m_si W_s] on v_mfma_i32_16x16x64_i8:
 // the codes give h (0/1/2) and m (0/1) as exact int8 K slots, and U, W are
 // split into three int8 slices of a fixed-point form with one power-of-two
 // scale per (SNP split, column):  U = s (q0 + q1 2^-7 + q2 2^-14) + e with
@@ -318,7 +118,8 @@ __device__ __forceinline__ void slices3(double x, double inv_s, int &q0, int &q1
 }
 
 // smax[split * kp + c] = max over rows of the split of f_s |T_sc|, with
-// f_s = |d_s| max(1, |mu_s|) (covers |U| = |d T| and |W| = |d mu T|).
+// f_s = |d_s| max(1, |mu_s|) (covers |U| = |d T| and |W| = |d mu T|), or
+// f_s = 1 when dsc is null.
 // Block: 4 x kp threads over 1024 rows of one split; one atomic per column.
 // smax must hold zeros (or smaller maxima) on entry.
 __global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
@@ -334,7 +135,7 @@ __global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
     if (r1 > rows) r1 = rows;
     double m = 0;
     for (int64_t s = r0 + rl; s < r1; s += 4) {
-        const double f = fabs(dsc[s]) * fmax(1.0, fabs(mu[s]));
+        const double f = dsc ? fabs(dsc[s]) * fmax(1.0, fabs(mu[s])) : 1.0;
         m = fmax(m, f * fabs(T[s * kp + c]));
     }
     float f = (float)m;
@@ -552,6 +353,200 @@ hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int
     const u32x4 *uw = reinterpret_cast<const u32x4 *>(UW8);
     if (sa_tiles == 4) return zt_i8_ct<4>(st, grid, packed, ldp, uw, smax, kp, rows, rows_per_split, Mpad, part);
     if (sa_tiles == 2) return zt_i8_ct<2>(st, grid, packed, ldp, uw, smax, kp, rows, rows_per_split, Mpad, part);
+    return hipErrorInvalidValue;
+}
+
+// ========================================================= Z Q on int8 MFMA ==
+// T = Z Q with T_s = d_s (sum_i h_si Q_i - mu_s sum_i m_si Q_i): the codes
+// give two exact int8 rows per SNP (h and m) and Q is split into three int8
+// slices with one power-of-two scale per (sample segment, column), so each
+// segment's sums are exact in int32 (|sum| <= 254 * 127 per 64 samples);
+// the slices, the two rows and the per-SNP factors are combined in FP64 at
+// the end of the segment and written as an FP32 partial of T.
+//
+// Q8 layout: per 16-sample group G, per slice j, per column c one 16-byte
+// word (byte e = sample 16G + e) at Q8[(G * 3 + j) * kp + c]; qmax[seg * kp
+// + c] the column's max |Q| over the segment.
+__global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int64_t seg_samples,
+                     const unsigned *__restrict__ qmax, u32x4 *__restrict__ Q8)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Mpad / 16 * kp) return;
+    const int64_t G = idx / kp;
+    const int c = (int)(idx % kp);
+    const int64_t seg = (G * 16) / seg_samples;
+    const double inv = 1.0 / slice_scale(__uint_as_float(qmax[seg * kp + c]));
+    uint32_t w[3][4] = {};
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+        int q0, q1, q2;
+        slices3(Q[(G * 16 + e) * kp + c], inv, q0, q1, q2);
+        const int sh = 8 * (e & 3), wd = e >> 2;
+        w[0][wd] |= (uint32_t)(q0 & 0xFF) << sh;
+        w[1][wd] |= (uint32_t)(q1 & 0xFF) << sh;
+        w[2][wd] |= (uint32_t)(q2 & 0xFF) << sh;
+    }
+#pragma unroll
+    for (int j = 0; j < 3; j++) Q8[(G * 3 + j) * kp + c] = u32x4{w[j][0], w[j][1], w[j][2], w[j][3]};
+}
+
+hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int kp, int64_t seg_bytes,
+                     unsigned *qmax, void *Q8)
+{
+    const int64_t segs = 4 * seg_bytes;  // samples per segment
+    if (Mpad % 256 || segs % 256 || kp * 4 > 256) return hipErrorInvalidValue;
+    const int64_t nseg = (Mpad + segs - 1) / segs;
+    hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(qmax), nseg * kp * 4, 0);
+    if (e != hipSuccess) return e;
+    dim3 g1((unsigned)((segs + 1023) / 1024), (unsigned)nseg);
+    k_split_max<<<g1, 4 * kp, 0, st>>>(Q, Mpad, kp, nullptr, nullptr, segs, qmax);
+    const int64_t n = Mpad / 16 * kp;
+    k_q8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Q, Mpad, kp, segs, qmax,
+                                                       reinterpret_cast<u32x4 *>(Q8));
+    return hipGetLastError();
+}
+
+// Workgroup: 8 waves x ST tiles x 16 SNPs, one sample segment (blockIdx.y).
+// Per 256-sample chunk (64 packed bytes per row), lane group g owns the 64
+// samples of its 16 packed bytes; at K step t it supplies samples
+// 64g + 16t .. +15.  The chunk's Q8 words are staged in LDS.
+template <int ST, int CT>
+__global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packed, int64_t ldp,
+                                               const u32x4 *__restrict__ Q8,
+                                               const unsigned *__restrict__ qmax,
+                                               const double *__restrict__ mu,
+                                               const double *__restrict__ dsc, int64_t seg_bytes,
+                                               int64_t Npad, float *__restrict__ Tpart)
+{
+    constexpr int KP = CT * 16;
+    constexpr int NS = 3;
+    constexpr int CB = 64;                      // packed bytes per chunk
+    constexpr int NQT = (CB / 4) * NS * KP;     // Q8 words per chunk
+    constexpr int NQ = (NQT + 511) / 512;
+    __shared__ u32x4 qs[NQT];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int64_t base = (int64_t)blockIdx.x * (8 * ST * 16) + wave * (ST * 16);
+    const int64_t b0 = (int64_t)blockIdx.y * seg_bytes;
+    int64_t b1 = b0 + seg_bytes;
+    if (b1 > ldp) b1 = ldp;
+    const uint32_t HT = 0x02010000u, MT = 0x01010001u;  // h, m by 2-bit code
+
+    i32x4 acch[ST][CT][NS], accm[ST][CT][NS];
+#pragma unroll
+    for (int s = 0; s < ST; s++)
+#pragma unroll
+        for (int c = 0; c < CT; c++)
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                acch[s][c][j] = i32x4{0, 0, 0, 0};
+                accm[s][c][j] = i32x4{0, 0, 0, 0};
+            }
+
+    u32x4 rq[NQ], rp[ST];
+#define DPCA_ZA8_LOAD(c0)                                                                   \
+    do {                                                                                    \
+        const u32x4 *src_ = Q8 + ((c0) / 4) * NS * KP;                                      \
+        _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
+            const int it_ = tid + 512 * u;                                                  \
+            if (it_ < NQT) rq[u] = src_[it_];                                               \
+        }                                                                                   \
+        _Pragma("unroll") for (int s = 0; s < ST; s++) rp[s] =                             \
+            *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + (c0) +     \
+                                             16 * g);                                       \
+    } while (0)
+    DPCA_ZA8_LOAD(b0 < b1 ? b0 : 0);
+
+    for (int64_t cb = b0; cb < b1; cb += CB) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < NQ; u++) {
+            const int it = tid + 512 * u;
+            if (it < NQT) qs[it] = rq[u];
+        }
+        u32x4 pk[ST];
+#pragma unroll
+        for (int s = 0; s < ST; s++) pk[s] = rp[s];
+        __syncthreads();
+        {
+            const int64_t nx = (cb + CB < b1) ? cb + CB : cb;
+            DPCA_ZA8_LOAD(nx);
+        }
+#pragma unroll 1
+        for (int t = 0; t < 4; t++) {
+            u32x4 b[NS][CT];
+#pragma unroll
+            for (int j = 0; j < NS; j++)
+#pragma unroll
+                for (int c = 0; c < CT; c++) b[j][c] = qs[((4 * g + t) * NS + j) * KP + c * 16 + r];
+#pragma unroll
+            for (int s = 0; s < ST; s++) {
+                const uint32_t w = pk[s][t];
+                u32x4 ah, am;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const uint32_t cw = spread4((w >> (8 * k)) & 0xFFu);
+                    ah[k] = __builtin_amdgcn_perm(0u, HT, cw);
+                    am[k] = __builtin_amdgcn_perm(0u, MT, cw);
+                }
+#pragma unroll
+                for (int j = 0; j < NS; j++)
+#pragma unroll
+                    for (int c = 0; c < CT; c++) {
+                        acch[s][c][j] = mfma_i8(ah, b[j][c], acch[s][c][j]);
+                        accm[s][c][j] = mfma_i8(am, b[j][c], accm[s][c][j]);
+                    }
+            }
+        }
+    }
+#undef DPCA_ZA8_LOAD
+    float *out = Tpart + (int64_t)blockIdx.y * Npad * KP;
+#pragma unroll
+    for (int c = 0; c < CT; c++) {
+        const double sc = slice_scale(__uint_as_float(qmax[blockIdx.y * KP + c * 16 + r]));
+#pragma unroll
+        for (int s = 0; s < ST; s++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int64_t row = base + s * 16 + 4 * g + q;
+                const double h = (double)acch[s][c][0][q] + ldexp((double)acch[s][c][1][q], -7) +
+                                 ldexp((double)acch[s][c][2][q], -14);
+                const double m = (double)accm[s][c][0][q] + ldexp((double)accm[s][c][1][q], -7) +
+                                 ldexp((double)accm[s][c][2][q], -14);
+                out[row * KP + c * 16 + r] = (float)(dsc[row] * sc * (h - mu[row] * m));
+            }
+    }
+}
+
+template <int ST>
+static hipError_t za_i8_ct(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
+                           const u32x4 *Q8, const unsigned *qmax, const double *mu,
+                           const double *dsc, int kp, int64_t seg_bytes, int64_t rows,
+                           float *Tpart)
+{
+    switch (kp) {
+    case 16: k_za_i8<ST, 1><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
+    case 32: k_za_i8<ST, 2><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
+    case 48: k_za_i8<ST, 3><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
+    case 64: k_za_i8<ST, 4><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// rows a multiple of 8*16*st_tiles, seg_bytes of 64, segments of at most
+// 65536 samples (int32 sums).
+hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                        const void *Q8, const unsigned *qmax, const double *mu,
+                        const double *dsc, int kp, int st_tiles, int64_t seg_bytes, int nseg,
+                        float *Tpart)
+{
+    const int64_t per = 8 * 16 * (int64_t)st_tiles;
+    if (rows % per || seg_bytes % 64 || ldp % 64 || seg_bytes > 16384) return hipErrorInvalidValue;
+    dim3 grid((unsigned)(rows / per), (unsigned)nseg);
+    const u32x4 *q = reinterpret_cast<const u32x4 *>(Q8);
+    if (st_tiles == 2) return za_i8_ct<2>(st, grid, packed, ldp, q, qmax, mu, dsc, kp, seg_bytes, rows, Tpart);
+    if (st_tiles == 1) return za_i8_ct<1>(st, grid, packed, ldp, q, qmax, mu, dsc, kp, seg_bytes, rows, Tpart);
     return hipErrorInvalidValue;
 }
 

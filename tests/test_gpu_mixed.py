@@ -1,5 +1,6 @@
-"""GPU checks of the mixed-precision form (bf16 hi+lo MFMA, FP32 accumulate,
-FP64 reductions and small dense algebra) against the FP64 oracle.
+"""GPU checks of the mixed-precision form (exact int8 codes x three int8
+slices of the FP64 operand on int8 MFMA, int32 accumulation, FP64
+reductions and small dense algebra) against the FP64 oracle.
 
 Stated tolerances: one pass within 2e-5 of the pass norm; for the principal
 components whose Ritz values have converged in the FP64 run (the population
