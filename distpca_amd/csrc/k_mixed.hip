@@ -198,8 +198,9 @@ hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int kp, con
 
 // Workgroup: 8 waves x SA tiles x 16 samples (SA = 4: 512 samples = 128
 // packed bytes per row) and one SNP split (blockIdx.y); 128-SNP chunks of
-// packed bytes staged in LDS transposed to [byte column][4-SNP group] as in
-// k_zt_mx, and the chunk's UW8 words copied to LDS.
+// packed bytes staged in LDS transposed to [byte column][4-SNP group], so a
+// lane reads the codes of 4 consecutive SNPs for its sample with one dword
+// load, and the chunk's UW8 words copied to LDS.
 template <int SA, int CT>
 __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ UW8,
@@ -212,8 +213,10 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
     constexpr int TB = SA * 16 * 8 / 4;    // packed bytes per row in this tile
     constexpr int SC = 128;                // SNPs per chunk
     constexpr int NUWT = (SC / 8) * NS * KP;  // UW8 words per chunk
-    __shared__ uint32_t pk[TB][SC / 4 + 1];
-    __shared__ u32x4 uw[NUWT];
+    // two buffers: chunk i is multiplied out of one while chunk i+1 lands in
+    // the other (one barrier per chunk)
+    __shared__ uint32_t pk[2][TB][SC / 4 + 1];
+    __shared__ u32x4 uw[2][NUWT];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
     const int64_t mb = (int64_t)blockIdx.x * TB;
@@ -233,6 +236,8 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
     constexpr int NPK = (SC / 4) * (TB / 4) / 512;   // 4x4 byte blocks per thread
     constexpr int NUW = (NUWT + 511) / 512;          // UW8 words per thread
     static_assert((SC / 4) * (TB / 4) % 512 == 0, "tile");
+    // software pipeline: the next chunk is loaded into registers while the
+    // current one is multiplied, then stored into the other LDS buffer
     uint32_t rpk[NPK][4];
     u32x4 ruw[NUW];
 #define DPCA_ZT8_LOAD(c0)                                                                   \
@@ -250,31 +255,35 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             if (it_ < NUWT) ruw[u] = q_[it_];                                               \
         }                                                                                   \
     } while (0)
+    // 4x4 byte transpose of each block into [byte column][SNP group]
+#define DPCA_ZT8_STORE(buf)                                                                 \
+    do {                                                                                    \
+        _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
+            const int it_ = tid + 512 * u;                                                  \
+            const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
+            const uint32_t r0_ = rpk[u][0], r1_ = rpk[u][1], r2_ = rpk[u][2], r3_ = rpk[u][3]; \
+            const uint32_t lo01_ = __builtin_amdgcn_perm(r1_, r0_, 0x05010400u);            \
+            const uint32_t hi01_ = __builtin_amdgcn_perm(r1_, r0_, 0x07030602u);            \
+            const uint32_t lo23_ = __builtin_amdgcn_perm(r3_, r2_, 0x05010400u);            \
+            const uint32_t hi23_ = __builtin_amdgcn_perm(r3_, r2_, 0x07030602u);            \
+            pk[buf][4 * bq_ + 0][sg_] = __builtin_amdgcn_perm(lo23_, lo01_, 0x05040100u);   \
+            pk[buf][4 * bq_ + 1][sg_] = __builtin_amdgcn_perm(lo23_, lo01_, 0x07060302u);   \
+            pk[buf][4 * bq_ + 2][sg_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x05040100u);   \
+            pk[buf][4 * bq_ + 3][sg_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x07060302u);   \
+        }                                                                                   \
+        _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
+            const int it_ = tid + 512 * u;                                                  \
+            if (it_ < NUWT) uw[buf][it_] = ruw[u];                                          \
+        }                                                                                   \
+    } while (0)
     DPCA_ZT8_LOAD(sbeg < send ? sbeg : 0);
+    DPCA_ZT8_STORE(0);
+    __syncthreads();
 
-    for (int64_t s0 = sbeg; s0 < send; s0 += SC) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NPK; u++) {
-            const int it = tid + 512 * u;
-            const int sg = it / (TB / 4), bq = it % (TB / 4);
-            const uint32_t r0 = rpk[u][0], r1 = rpk[u][1], r2 = rpk[u][2], r3 = rpk[u][3];
-            const uint32_t lo01 = __builtin_amdgcn_perm(r1, r0, 0x05010400u);
-            const uint32_t hi01 = __builtin_amdgcn_perm(r1, r0, 0x07030602u);
-            const uint32_t lo23 = __builtin_amdgcn_perm(r3, r2, 0x05010400u);
-            const uint32_t hi23 = __builtin_amdgcn_perm(r3, r2, 0x07030602u);
-            pk[4 * bq + 0][sg] = __builtin_amdgcn_perm(lo23, lo01, 0x05040100u);
-            pk[4 * bq + 1][sg] = __builtin_amdgcn_perm(lo23, lo01, 0x07060302u);
-            pk[4 * bq + 2][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x05040100u);
-            pk[4 * bq + 3][sg] = __builtin_amdgcn_perm(hi23, hi01, 0x07060302u);
-        }
-#pragma unroll
-        for (int u = 0; u < NUW; u++) {
-            const int it = tid + 512 * u;
-            if (it < NUWT) uw[it] = ruw[u];
-        }
-        __syncthreads();
+    int cur = 0;
+    for (int64_t s0 = sbeg; s0 < send; s0 += SC, cur ^= 1) {
         {
+            // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
             DPCA_ZT8_LOAD(nx);
         }
@@ -285,8 +294,8 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             for (int a = 0; a < SA; a++) {
                 const int i = wave * SA * 16 + a * 16 + r;  // sample within the tile
                 const int sh = 2 * (i & 3);
-                const uint32_t c0 = (pk[i >> 2][8 * t + 2 * g] >> sh) & 0x03030303u;
-                const uint32_t c1 = (pk[i >> 2][8 * t + 2 * g + 1] >> sh) & 0x03030303u;
+                const uint32_t c0 = (pk[cur][i >> 2][8 * t + 2 * g] >> sh) & 0x03030303u;
+                const uint32_t c1 = (pk[cur][i >> 2][8 * t + 2 * g + 1] >> sh) & 0x03030303u;
                 av[a] = u32x4{__builtin_amdgcn_perm(0u, HT, c0), __builtin_amdgcn_perm(0u, HT, c1),
                               __builtin_amdgcn_perm(0u, MT, c0), __builtin_amdgcn_perm(0u, MT, c1)};
             }
@@ -294,14 +303,17 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             for (int j = 0; j < NS; j++) {
                 u32x4 b[CT];
 #pragma unroll
-                for (int c = 0; c < CT; c++) b[c] = uw[((4 * t + g) * NS + j) * KP + c * 16 + r];
+                for (int c = 0; c < CT; c++) b[c] = uw[cur][((4 * t + g) * NS + j) * KP + c * 16 + r];
 #pragma unroll
                 for (int a = 0; a < SA; a++)
 #pragma unroll
                     for (int c = 0; c < CT; c++) acc[a][c][j] = mfma_i8(av[a], b[c], acc[a][c][j]);
             }
         }
+        DPCA_ZT8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
+        __syncthreads();
     }
+#undef DPCA_ZT8_STORE
 #undef DPCA_ZT8_LOAD
     float *out = part + ((int64_t)blockIdx.y * Mpad + 4 * mb + wave * SA * 16) * KP;
 #pragma unroll
@@ -423,7 +435,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     constexpr int CB = 64;                      // packed bytes per chunk
     constexpr int NQT = (CB / 4) * NS * KP;     // Q8 words per chunk
     constexpr int NQ = (NQT + 511) / 512;
-    __shared__ u32x4 qs[NQT];
+    __shared__ u32x4 qs[2][NQT];               // double buffer, one barrier per chunk
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
     const int64_t base = (int64_t)blockIdx.x * (8 * ST * 16) + wave * (ST * 16);
@@ -443,6 +455,9 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
                 accm[s][c][j] = i32x4{0, 0, 0, 0};
             }
 
+    // software pipeline: the next chunk's Q8 words and packed bytes are
+    // loaded into registers while the current chunk is multiplied; the Q8
+    // words then go to the other LDS buffer
     u32x4 rq[NQ], rp[ST];
 #define DPCA_ZA8_LOAD(c0)                                                                   \
     do {                                                                                    \
@@ -455,19 +470,22 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
             *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + (c0) +     \
                                              16 * g);                                       \
     } while (0)
+#define DPCA_ZA8_STORE(buf)                                                                 \
+    do {                                                                                    \
+        _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
+            const int it_ = tid + 512 * u;                                                  \
+            if (it_ < NQT) qs[buf][it_] = rq[u];                                            \
+        }                                                                                   \
+    } while (0)
     DPCA_ZA8_LOAD(b0 < b1 ? b0 : 0);
+    DPCA_ZA8_STORE(0);
+    __syncthreads();
 
-    for (int64_t cb = b0; cb < b1; cb += CB) {
-        __syncthreads();
-#pragma unroll
-        for (int u = 0; u < NQ; u++) {
-            const int it = tid + 512 * u;
-            if (it < NQT) qs[it] = rq[u];
-        }
+    int cur = 0;
+    for (int64_t cb = b0; cb < b1; cb += CB, cur ^= 1) {
         u32x4 pk[ST];
 #pragma unroll
         for (int s = 0; s < ST; s++) pk[s] = rp[s];
-        __syncthreads();
         {
             const int64_t nx = (cb + CB < b1) ? cb + CB : cb;
             DPCA_ZA8_LOAD(nx);
@@ -478,7 +496,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 #pragma unroll
             for (int j = 0; j < NS; j++)
 #pragma unroll
-                for (int c = 0; c < CT; c++) b[j][c] = qs[((4 * g + t) * NS + j) * KP + c * 16 + r];
+                for (int c = 0; c < CT; c++) b[j][c] = qs[cur][((4 * g + t) * NS + j) * KP + c * 16 + r];
 #pragma unroll
             for (int s = 0; s < ST; s++) {
                 const uint32_t w = pk[s][t];
@@ -498,7 +516,10 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
                     }
             }
         }
+        DPCA_ZA8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
+        __syncthreads();
     }
+#undef DPCA_ZA8_STORE
 #undef DPCA_ZA8_LOAD
     float *out = Tpart + (int64_t)blockIdx.y * Npad * KP;
 #pragma unroll
@@ -528,7 +549,13 @@ static hipError_t za_i8_ct(hipStream_t st, dim3 grid, const uint8_t *packed, int
     case 16: k_za_i8<ST, 1><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
     case 32: k_za_i8<ST, 2><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
     case 48: k_za_i8<ST, 3><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
-    case 64: k_za_i8<ST, 4><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
+    case 64:
+        // 2 tiles x 4 column tiles x 3 slices x 2 rows would not fit the registers
+        if constexpr (ST <= 1) {
+            k_za_i8<ST, 4><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart);
+            break;
+        }
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
