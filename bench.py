@@ -154,7 +154,13 @@ def serve_bench(a, D):
             with cond:
                 q.append(item)
                 cond.notify_all()
-        th.join()
+            if (i + 1) % 50 == 0:
+                log(f"serve[{form_name}]: {i + 1}/{n_req} submitted")
+        while th.is_alive():
+            th.join(timeout=30)
+            if th.is_alive():
+                log(f"serve[{form_name}]: draining, {len(lat['project']) + len(lat['job'])}"
+                    f"/{n_req} done")
         wall = time.perf_counter() - t0
         cnt = srv.counters()
         srv.close()
