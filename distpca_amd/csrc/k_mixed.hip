@@ -287,7 +287,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
             DPCA_ZT8_LOAD(nx);
         }
-#pragma unroll 1
+#pragma unroll
         for (int t = 0; t < SC / 32; t++) {
             u32x4 av[SA];
 #pragma unroll
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
             const int64_t nx = (cb + CB < b1) ? cb + CB : cb;
             DPCA_ZA8_LOAD(nx);
         }
-#pragma unroll 1
+#pragma unroll
         for (int t = 0; t < 4; t++) {
             u32x4 b[NS][CT];
 #pragma unroll
