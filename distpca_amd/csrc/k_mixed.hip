@@ -214,8 +214,15 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
     constexpr int SC = 128;                // SNPs per chunk
     constexpr int NUWT = (SC / 8) * NS * KP;  // UW8 words per chunk
     // two buffers: chunk i is multiplied out of one while chunk i+1 lands in
-    // the other (one barrier per chunk)
-    __shared__ uint32_t pk[2][TB][SC / 4 + 1];
+    // the other (one barrier per chunk).  pk rows are byte columns, their
+    // dwords 4-SNP groups, stored at column sg ^ (row / 4) in rows of 36
+    // dwords: both the transposing stores (one group, many rows per
+    // half-wave) and the paired reads (4 rows x 2 groups) then touch
+    // distinct banks.  Groups 2x, 2x+1 stay one aligned 8-byte pair, in
+    // swapped order when row / 4 is odd.
+    constexpr int PKW = SC / 4 + 4;
+    static_assert(SA % 2 == 0, "row / 4 parity must follow the tile index");
+    __shared__ uint32_t pk[2][TB][PKW];
     __shared__ u32x4 uw[2][NUWT];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
@@ -266,10 +273,11 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             const uint32_t hi01_ = __builtin_amdgcn_perm(r1_, r0_, 0x07030602u);            \
             const uint32_t lo23_ = __builtin_amdgcn_perm(r3_, r2_, 0x05010400u);            \
             const uint32_t hi23_ = __builtin_amdgcn_perm(r3_, r2_, 0x07030602u);            \
-            pk[buf][4 * bq_ + 0][sg_] = __builtin_amdgcn_perm(lo23_, lo01_, 0x05040100u);   \
-            pk[buf][4 * bq_ + 1][sg_] = __builtin_amdgcn_perm(lo23_, lo01_, 0x07060302u);   \
-            pk[buf][4 * bq_ + 2][sg_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x05040100u);   \
-            pk[buf][4 * bq_ + 3][sg_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x07060302u);   \
+            const int col_ = sg_ ^ (bq_ & 31);                                              \
+            pk[buf][4 * bq_ + 0][col_] = __builtin_amdgcn_perm(lo23_, lo01_, 0x05040100u);  \
+            pk[buf][4 * bq_ + 1][col_] = __builtin_amdgcn_perm(lo23_, lo01_, 0x07060302u);  \
+            pk[buf][4 * bq_ + 2][col_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x05040100u);  \
+            pk[buf][4 * bq_ + 3][col_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x07060302u);  \
         }                                                                                   \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
             const int it_ = tid + 512 * u;                                                  \
@@ -294,8 +302,13 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             for (int a = 0; a < SA; a++) {
                 const int i = wave * SA * 16 + a * 16 + r;  // sample within the tile
                 const int sh = 2 * (i & 3);
-                const uint32_t c0 = (pk[cur][i >> 2][8 * t + 2 * g] >> sh) & 0x03030303u;
-                const uint32_t c1 = (pk[cur][i >> 2][8 * t + 2 * g + 1] >> sh) & 0x03030303u;
+                const int sw = (wave * SA + a) & 30;         // (i / 16) & ~1
+                const uint2 w2 = *reinterpret_cast<const uint2 *>(
+                    &pk[cur][i >> 2][(8 * t + 2 * g) ^ sw]);
+                const uint32_t x0 = (a & 1) ? w2.y : w2.x;   // group 8t + 2g
+                const uint32_t x1 = (a & 1) ? w2.x : w2.y;   // group 8t + 2g + 1
+                const uint32_t c0 = (x0 >> sh) & 0x03030303u;
+                const uint32_t c1 = (x1 >> sh) & 0x03030303u;
                 av[a] = u32x4{__builtin_amdgcn_perm(0u, HT, c0), __builtin_amdgcn_perm(0u, HT, c1),
                               __builtin_amdgcn_perm(0u, MT, c0), __builtin_amdgcn_perm(0u, MT, c1)};
             }
