@@ -84,6 +84,11 @@ class ServerCounters(ctypes.Structure):
 
 TICKET_PROJECT, TICKET_JOB = 1, 2
 
+class FullSvdStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("frob_norm_angle", ctypes.c_double),
+                ("cos_error", ctypes.c_double), ("info", ctypes.c_int)]
+
+
 EXPORTS = [
     "dpca_params_init", "dpca_open", "dpca_open_packed", "dpca_open_synthetic",
     "dpca_write_synthetic", "dpca_get_dims", "dpca_omega", "dpca_snp_stats", "dpca_decode",
@@ -93,6 +98,7 @@ EXPORTS = [
     "dpca_model_free", "dpca_server_params_init", "dpca_server_create",
     "dpca_server_submit_project", "dpca_server_submit_job", "dpca_server_wait",
     "dpca_server_stats", "dpca_server_destroy", "dpca_host_alloc", "dpca_host_free",
+    "dpca_full_svd", "dpca_write_fullsvd",
 ]
 
 _lib = None
@@ -146,6 +152,8 @@ def lib():
     L.dpca_host_alloc.restype = vp
     L.dpca_host_free.argtypes = [vp]
     L.dpca_host_free.restype = None
+    L.dpca_full_svd.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, P(FullSvdStats)]
+    L.dpca_write_fullsvd.argtypes = [ctypes.c_char_p, i32, i32, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -305,6 +313,23 @@ class PCA:
         buf = np.empty(1024, np.float64)
         n = lib().dpca_crit_history(self._h, _ptr(buf), 1024)
         return buf[:n].copy()
+
+    def full_svd(self, rhs: np.ndarray, sigma: np.ndarray):
+        """-fullSVD diagnostic against a finished run: dict with true_sigma,
+        true_U (M x nsv), sigma_relerr, vec_relerr, cos_values and the stats."""
+        M, nsv = self.M, self.nsv
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        out = {"true_sigma": np.empty(nsv), "true_U": np.empty((M, nsv)),
+               "sigma_relerr": np.empty(nsv), "vec_relerr": np.empty((M, nsv)),
+               "cos_values": np.empty(nsv)}
+        st = FullSvdStats()
+        _check(lib().dpca_full_svd(self._h, _ptr(rhs), _ptr(sigma), _ptr(out["true_sigma"]),
+                                   _ptr(out["true_U"]), _ptr(out["sigma_relerr"]),
+                                   _ptr(out["vec_relerr"]), _ptr(out["cos_values"]),
+                                   ctypes.byref(st)))
+        out.update({f: getattr(st, f) for f, _ in st._fields_})
+        return out
 
     def write_outputs(self, prefix: str, rhs: np.ndarray, sigma: np.ndarray):
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)

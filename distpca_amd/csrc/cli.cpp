@@ -63,7 +63,7 @@ int ram_kb()
 // Console report of print_statistics (src/utilities.cpp:147-193).  The
 // scripts read field 5 of the first "Total wall-clock time elapsed:" line of
 // the seconds block; as in the reference it is the sum of the stage timers.
-void print_statistics(const dpca_report &r, int threads, int rkb)
+void print_statistics(const dpca_report &r, int threads, int rkb, bool fullsvd)
 {
     time_t now = time(nullptr);
     printf("----------------------------------------------------------\n");
@@ -102,6 +102,13 @@ void print_statistics(const dpca_report &r, int threads, int rkb)
         printf("Total wall-clock time elapsed (without including the %s spent on loading the "
                "matrix): %02.13f\n",
                u == 0 ? "amount of time" : "time", tot_noload / d);
+    }
+    // The reference prints these lines before it computes the full SVD
+    // (src/distpca.cpp:446-449), so they always carry the initial zeros.
+    if (r.incore && fullsvd) {
+        printf("\nTotal wall-clock time to compute the full ('econ') SVD: %02.13f\n", 0.0);
+        printf("||Uhat^TU - I||_F: %02.13f\n", 0.0);
+        printf("||Uhat^TU - I||_2: %02.13f\n", 0.0);
     }
 }
 
@@ -216,15 +223,35 @@ int main(int argc, char **argv)
     }
     if (p.verbose > 0) {
         const char *omp = getenv("OMP_NUM_THREADS");
-        print_statistics(rep, omp ? atoi(omp) : 1, rkb);
+        print_statistics(rep, omp ? atoi(omp) : 1, rkb, p.fullsvd == 1);
     }
     if (p.fwrite == 1 && dpca_write_outputs(ctx, p.prefix, rhs.data(), sigma.data()) != 0) {
         printf("\n%s\n", dpca_last_error());
         dpca_close(ctx);
         return 1;
     }
-    if (p.fullsvd == 1)
-        printf("\n-fullSVD: the full-SVD accuracy diagnostic is not part of this build\n");
+    if (p.fullsvd == 1) {
+        // accuracy diagnostic of src/distpca.cpp:449-635
+        std::vector<double> ts(nsv), tu((size_t)M * nsv), se(nsv), ve((size_t)M * nsv), cv(nsv);
+        dpca_fullsvd_stats fs;
+        if (dpca_full_svd(ctx, rhs.data(), sigma.data(), ts.data(), tu.data(), se.data(),
+                          ve.data(), cv.data(), &fs) != 0) {
+            printf("\n%s\n", dpca_last_error());
+            dpca_close(ctx);
+            return 1;
+        }
+        if (p.verbose > 1) {
+            printf("\nFULL SVD computed in %lf sec, info = %d\n\n", fs.seconds, fs.info);
+            for (int j = 0; j < nsv; j++) printf("RelError Sigma(%d): %e\n", j, se[j]);
+            for (int j = 0; j < nsv; j++) printf("Cosine(%d): %lf\n", j, cv[j]);
+        }
+        if (p.fwrite == 1 &&
+            dpca_write_fullsvd(p.prefix, M, nsv, tu.data(), ts.data(), ve.data(), cv.data()) != 0) {
+            printf("%s\n", dpca_last_error());
+            dpca_close(ctx);
+            return 1;
+        }
+    }
     dpca_close(ctx);
     return 0;
 }

@@ -1663,6 +1663,149 @@ int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms)
     return 0;
 }
 
+// Largest singular value of the k x k row-major matrix C: square root of the
+// largest eigenvalue of C^T C by cyclic Jacobi (k <= 64).
+static double largest_singular_value(const std::vector<double> &C, int k)
+{
+    std::vector<double> A((size_t)k * k, 0.0);
+    for (int i = 0; i < k; i++)
+        for (int j = 0; j < k; j++) {
+            double t = 0;
+            for (int r = 0; r < k; r++) t += C[(size_t)r * k + i] * C[(size_t)r * k + j];
+            A[(size_t)i * k + j] = t;
+        }
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0, diag = 0;
+        for (int i = 0; i < k; i++)
+            for (int j = 0; j < k; j++)
+                (i == j ? diag : off) += A[(size_t)i * k + j] * A[(size_t)i * k + j];
+        if (off <= 1e-30 * diag) break;
+        for (int p = 0; p < k - 1; p++)
+            for (int q = p + 1; q < k; q++) {
+                const double apq = A[(size_t)p * k + q];
+                if (apq == 0) continue;
+                const double tau = (A[(size_t)q * k + q] - A[(size_t)p * k + p]) / (2 * apq);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+                const double cs = 1 / sqrt(1 + t * t), sn = t * cs;
+                for (int r = 0; r < k; r++) {  // A <- J^T A J
+                    const double x = A[(size_t)r * k + p], y = A[(size_t)r * k + q];
+                    A[(size_t)r * k + p] = cs * x - sn * y;
+                    A[(size_t)r * k + q] = sn * x + cs * y;
+                }
+                for (int r = 0; r < k; r++) {
+                    const double x = A[(size_t)p * k + r], y = A[(size_t)q * k + r];
+                    A[(size_t)p * k + r] = cs * x - sn * y;
+                    A[(size_t)q * k + r] = sn * x + cs * y;
+                }
+            }
+    }
+    double m = 0;
+    for (int i = 0; i < k; i++) m = std::max(m, A[(size_t)i * k + i]);
+    return sqrt(std::max(m, 0.0));
+}
+
+int dpca_full_svd(dpca_ctx *c, const double *rhs, const double *sigma, double *true_sigma,
+                  double *true_U, double *sigma_relerr, double *vec_relerr, double *cos_values,
+                  dpca_fullsvd_stats *st)
+{
+    if (!c || !rhs || !sigma || !true_sigma || !true_U || !sigma_relerr || !vec_relerr ||
+        !cos_values || !st) {
+        set_error("dpca_full_svd: NULL argument");
+        return -1;
+    }
+    DPCA_CHECK(hipSetDevice(c->dev));
+    const int M = c->M, k = c->nsv;
+    const int64_t N = c->N;
+    int r;
+    if (!c->stats_done && (r = stream_stats_sweep(c))) return r;
+    const double t0 = now_s();
+    double *B = nullptr;
+    if (dalloc(&B, (size_t)N * M * sizeof(double))) {
+        set_error("dpca_full_svd: the dense %d x %lld FP64 matrix (%.1f GB) does not fit on the device",
+                  M, (long long)N, (double)N * M * 8 / 1e9);
+        return -3;
+    }
+    hipError_t e = hipSuccess;
+    r = 0;
+    if (!c->streamed) {
+        e = launch_decode_f64(c->st, c->packed, N, c->ldp, M, c->zval, B);
+    } else {
+        for (int64_t r0 = 0; r0 < N && !r && e == hipSuccess; r0 += c->chunk_rows) {
+            const int64_t rows = std::min(c->chunk_rows, c->Npad - r0);
+            if ((r = fetch_block(c, 0, r0, rows, c->dbuf[0], c->st))) break;
+            e = launch_decode_f64(c->st, c->dbuf[0], std::min(rows, N - r0), c->ldp, M,
+                                  c->zval + 4 * r0, B + r0 * M);
+        }
+    }
+    int info = 0;
+    if (!r && e == hipSuccess) r = dense_top_svd(c->st, B, M, N, k, true_sigma, true_U, &info);
+    (void)hipFree(B);
+    if (e != hipSuccess) {
+        set_error("dpca_full_svd: %s", hipGetErrorString(e));
+        return -2;
+    }
+    if (r) return r;
+    st->seconds = now_s() - t0;
+    st->info = info;
+    // comparisons of src/distpca.cpp:510-570 and src/utilities.cpp:234-270
+    for (int j = 0; j < k; j++) {
+        const double den = true_sigma[j] ? true_sigma[j] : 1.0;
+        sigma_relerr[j] = fabs(true_sigma[j] - sigma[j]) / den;
+    }
+    std::vector<double> C((size_t)k * k, 0.0);
+    for (int i = 0; i < M; i++)
+        for (int a = 0; a < k; a++) {
+            const double uh = rhs[(size_t)i * c->k + a], ut = true_U[(size_t)i * k + a];
+            vec_relerr[(size_t)i * k + a] = fabs(fabs(ut) - fabs(uh)) / (fabs(ut) + 1e-12);
+            for (int b = 0; b < k; b++) C[(size_t)a * k + b] += uh * true_U[(size_t)i * k + b];
+        }
+    double frob = 0;
+    for (int a = 0; a < k; a++)
+        for (int b = 0; b < k; b++) {
+            const double x = C[(size_t)a * k + b];
+            frob += (a == b) ? (fabs(x) - 1.0) * (fabs(x) - 1.0) : x * x;
+        }
+    st->frob_norm_angle = sqrt(frob);
+    for (int j = 0; j < k; j++) {
+        cos_values[j] = C[(size_t)j * k + j];
+        C[(size_t)j * k + j] = fabs(C[(size_t)j * k + j]) - 1.0;
+    }
+    st->cos_error = largest_singular_value(C, k);
+    return 0;
+}
+
+int dpca_write_fullsvd(const char *prefix, int M, int nsv, const double *true_U,
+                       const double *true_sigma, const double *vec_relerr,
+                       const double *cos_values)
+{
+    if (!prefix || !true_U || !true_sigma || !vec_relerr || !cos_values || M < 1 || nsv < 1) {
+        set_error("dpca_write_fullsvd: bad arguments");
+        return -1;
+    }
+    const std::string p(prefix);
+    FILE *fvec = fopen((p + "_realLeftSingularVectors.txt").c_str(), "w");
+    FILE *fval = fopen((p + "_realSingularValues.txt").c_str(), "w");
+    FILE *ferr = fopen((p + "_singvecs_accuracy.txt").c_str(), "w");
+    FILE *fcos = fopen((p + "_cosineValues.txt").c_str(), "w");
+    int rc = 0;
+    if (!fvec || !fval || !ferr || !fcos) {
+        set_error("File error");
+        rc = -1;
+    } else {
+        for (int i = 0; i < M; i++) {
+            for (int j = 0; j < nsv; j++) fprintf(fvec, "%2.13f ", true_U[(size_t)i * nsv + j]);
+            fprintf(fvec, "\n");
+        }
+        for (int j = 0; j < nsv; j++) fprintf(fval, "%2.13f\n", true_sigma[j]);
+        for (int64_t e = 0; e < (int64_t)M * nsv; e++) fprintf(ferr, "%2.13lf\n", vec_relerr[e]);
+        for (int j = 0; j < nsv; j++) fprintf(fcos, "%2.13lf\n", cos_values[j]);
+    }
+    FILE *fs[] = {fvec, fval, ferr, fcos};
+    for (FILE *f : fs)
+        if (f) fclose(f);
+    return rc;
+}
+
 void dpca_close(dpca_ctx *c) { free_ctx(c); }
 
 int dpca_footprint(int M, long long N, const dpca_params *p, long long *bytes)

@@ -53,6 +53,15 @@ hipError_t launch_snp_stats(hipStream_t st, const uint8_t *packed, int64_t rows,
                             double *zval, double *mu, double *dsc);
 
 // Decode rows to reference codes {0,1,2,3=missing} (first-touch table).
+// Dense FP64 standardised rows (rows x M, SNP-major) for the full SVD.
+hipError_t launch_decode_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                             int M, const double *zval, double *out);
+// Top nsv eigenpairs of the Gram matrix of B (M x N, column-major, ld M,
+// on the device) through rocBLAS / rocSOLVER loaded at first use: sigma
+// (descending) and the left singular vectors U (M x nsv, row-major) of B,
+// both on the host; *info is the eigensolver's status.
+int dense_top_svd(hipStream_t st, const double *B, int M, int64_t N, int nsv, double *sigma,
+                  double *U, int *info);
 hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t rows,
                                int64_t ldp, int M, uint8_t *codes);
 

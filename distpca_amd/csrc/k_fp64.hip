@@ -265,6 +265,29 @@ hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t ro
     return hipGetLastError();
 }
 
+// Dense standardised matrix for the full-SVD diagnostic: out[s * M + i] =
+// zval[4 s + code(s, i)] for rows s < rows and samples i < M (the values
+// src/utilities.cpp:272-306 produce; SNP-major, i.e. column-major M x rows).
+__global__ void k_decode_f64(const uint8_t *__restrict__ packed, int64_t rows, int64_t ldp, int M,
+                             const double *__restrict__ zval, double *__restrict__ out)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * M) return;
+    const int64_t s = e / M;
+    const int i = (int)(e % M);
+    const int code = (packed[s * ldp + (i >> 2)] >> (2 * (i & 3))) & 3;
+    out[e] = zval[4 * s + code];
+}
+
+hipError_t launch_decode_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                             int M, const double *zval, double *out)
+{
+    const int64_t n = rows * M;
+    if (n == 0) return hipSuccess;
+    k_decode_f64<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(packed, rows, ldp, M, zval, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------- FP64  T = Z Q --
 // v_mfma_f64_16x16x4f64: D(16x16) += A(16x4) B(4x16).  Lane l = 16*g + r
 // supplies A[r][g] and B[g][r] and holds D[4q + g][r], q = 0..3 (rows

@@ -183,6 +183,30 @@ const char *dpca_last_error(void);
  * (no reference counterpart; used for HBM admission). */
 int dpca_footprint(int M, long long N, const dpca_params *p, long long *bytes);
 
+/* -fullSVD diagnostic (src/distpca.cpp:449-635, src/utilities.cpp:234-270):
+ * the leading nsv singular values and left singular vectors of the whole
+ * standardised matrix (from the eigendecomposition of its Gram matrix on the
+ * device, rocBLAS + rocSOLVER loaded at first use), compared with a finished
+ * run's sigma (nsv) and Ritz vectors rhs (M x nrhs, first nsv columns):
+ * true_sigma[nsv], true_U[M x nsv], sigma_relerr[j] = |true - sigma| / true,
+ * vec_relerr[i x nsv + j] = ||U_ij| - |Uhat_ij|| / (|U_ij| + 1e-12),
+ * cos_values[j] = (Uhat^T U)_jj.  Needs M x N x 8 bytes of device memory. */
+typedef struct dpca_fullsvd_stats {
+    double seconds;          /* TIME_2_TRUE_SVD                               */
+    double frob_norm_angle;  /* ||Uhat^T U - I||_F with |.| on the diagonal   */
+    double cos_error;        /* ||Uhat^T U - I||_2, same diagonal convention  */
+    int info;                /* eigensolver status, 0 = success               */
+} dpca_fullsvd_stats;
+int dpca_full_svd(dpca_ctx *c, const double *rhs, const double *sigma, double *true_sigma,
+                  double *true_U, double *sigma_relerr, double *vec_relerr, double *cos_values,
+                  dpca_fullsvd_stats *st);
+/* <prefix>_realLeftSingularVectors.txt, _realSingularValues.txt,
+ * _singvecs_accuracy.txt, _cosineValues.txt in the reference's formats
+ * (src/distpca.cpp:573-620). */
+int dpca_write_fullsvd(const char *prefix, int M, int nsv, const double *true_U,
+                       const double *true_sigma, const double *vec_relerr,
+                       const double *cos_values);
+
 /* ---- Projection and serving (no reference counterpart: DistPCA stops at
  * the left singular vectors, src/methods.cpp:1072-1118) ------------------ */
 

@@ -32,6 +32,11 @@ How each function is pinned (tests/test_oracle.py):
                              (example/hgdp.qc.pruned.log:40); converged
                              eigenvalues match a dense eigendecomposition of
                              Z^T Z.
+  full_svd_diag              the comparisons of -fullSVD: exact cases (the
+                             true vectors themselves, sign flips, a known
+                             plane rotation by theta: ||.||_F =
+                             2 sqrt(1 - cos theta), ||.||_2 = 2 sin(theta/2))
+                             and numpy's SVD reconstructing the matrix.
 """
 from __future__ import annotations
 
@@ -423,6 +428,29 @@ def rsi_incore(Z: np.ndarray, p: Params, rhs0: np.ndarray | None = None) -> Resu
 
 
 # ----------------------------------------------------------------- metrics --
+
+def full_svd_diag(Zt: np.ndarray, Uhat: np.ndarray, sigma: np.ndarray,
+                  U: np.ndarray | None = None, s: np.ndarray | None = None) -> dict:
+    """-fullSVD diagnostic (src/distpca.cpp:449-570, computeCosineError in
+    src/utilities.cpp:234-270): economy SVD of the M x N standardised matrix Zt
+    (dgesvd there, numpy here unless U, s are given), then the comparisons
+    with the run's nsv Ritz vectors Uhat (M x nsv) and sigma (nsv)."""
+    nsv = Uhat.shape[1]
+    if U is None:
+        U, s, _ = np.linalg.svd(Zt, full_matrices=False)
+    U, s = U[:, :nsv], s[:nsv]
+    den = np.where(s != 0, s, 1.0)
+    sig_err = np.abs(s - sigma) / den
+    vec_err = np.abs(np.abs(U) - np.abs(Uhat)) / (np.abs(U) + 1e-12)
+    C = Uhat.T @ U
+    d = np.diag(C).copy()
+    frob = math.sqrt(float(np.sum((np.abs(d) - 1.0) ** 2) + np.sum(C ** 2) - np.sum(d ** 2)))
+    C2 = C.copy()
+    np.fill_diagonal(C2, np.abs(d) - 1.0)
+    cos_err = float(np.linalg.svd(C2, compute_uv=False)[0])
+    return {"true_sigma": s, "true_U": U, "sigma_relerr": sig_err, "vec_relerr": vec_err,
+            "cos_values": d, "frob_norm_angle": frob, "cos_error": cos_err}
+
 
 def principal_angles(U: np.ndarray, V: np.ndarray) -> np.ndarray:
     """Principal angles (radians) between span(U) and span(V), ascending.

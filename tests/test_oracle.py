@@ -199,3 +199,29 @@ def test_golden_fixture_consistency(hgdp_golden, c1_golden):
     assert int(hgdp_golden["total_its"]) == 11
     assert hgdp_golden["crit"][-1] <= 1e-3 < hgdp_golden["crit"][-2]
     assert int(c1_golden["total_its"]) == 20
+
+
+def test_full_svd_diag_closed_forms():
+    # -fullSVD comparisons (src/distpca.cpp:510-570, src/utilities.cpp:234-270)
+    rng = np.random.default_rng(3)
+    Zt = rng.standard_normal((60, 400))
+    U, s, Vt = np.linalg.svd(Zt, full_matrices=False)
+    assert np.abs(U * s @ Vt - Zt).max() < 1e-12 * np.abs(Zt).max()
+    k = 4
+    d = O.full_svd_diag(Zt, U[:, :k], s[:k])
+    assert d["frob_norm_angle"] < 1e-12 and d["cos_error"] < 1e-12
+    assert np.allclose(d["cos_values"], 1) and d["sigma_relerr"].max() < 1e-14
+    # column signs do not matter except in the cosine values themselves
+    flip = U[:, :k] * np.array([1, -1, 1, -1])
+    d = O.full_svd_diag(Zt, flip, s[:k])
+    assert d["frob_norm_angle"] < 1e-12 and d["cos_error"] < 1e-12
+    assert np.allclose(d["cos_values"], [1, -1, 1, -1]) and d["vec_relerr"].max() < 1e-12
+    # a rotation by theta inside the plane of PCs 1 and 2
+    th = 0.3
+    Uh = U[:, :k].copy()
+    Uh[:, 0] = np.cos(th) * U[:, 0] - np.sin(th) * U[:, 1]
+    Uh[:, 1] = np.sin(th) * U[:, 0] + np.cos(th) * U[:, 1]
+    d = O.full_svd_diag(Zt, Uh, s[:k] * 1.01)
+    assert abs(d["frob_norm_angle"] - 2 * np.sqrt(1 - np.cos(th))) < 1e-12
+    assert abs(d["cos_error"] - 2 * np.sin(th / 2)) < 1e-12
+    assert np.allclose(d["sigma_relerr"], 0.01)
