@@ -381,19 +381,20 @@ static int alloc_work(dpca_ctx *c)
         const int64_t nsegs = (c->ldp + 4095) / 4096;
         c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 64);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
-        c->st_tiles = (c->kp <= 48) ? 2 : 1;
+        c->st_tiles = za_i8_tiles(c->k);
         if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 1, 2
         // Z^T T: SNP splits of <= 16384 rows, >= ~1024 workgroups
-        c->sa_tiles = zt_i8_tiles(c->kp, c->ldp);
+        c->sa_tiles = zt_i8_tiles(c->k, c->ldp);
         if (const char *e = getenv("DPCA_ZT_TILES")) c->sa_tiles = atoi(e);  // tuning knob: 2, 4
         const int64_t tl = c->ldp / (c->sa_tiles * 32);
         int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 16383) / 16384);
         ns = std::min<int64_t>(ns, B / 128);
         c->rps = round_up((B + ns - 1) / ns, 128);
         c->nsplit = (int)((B + c->rps - 1) / c->rps);
-        r |= dalloc(&c->Q8, (size_t)c->Mpad / 16 * 3 * c->kp * sizeof(uint4));
+        const int vt = col_tiles(c->k).vt();
+        r |= dalloc(&c->Q8, (size_t)c->Mpad / 16 * vt * 16 * sizeof(uint4));
         r |= dalloc(&c->qmax, (size_t)c->nseg * c->kp * sizeof(unsigned));
-        r |= dalloc(&c->UW8, (size_t)c->Npad / 8 * 3 * c->kp * sizeof(uint4));
+        r |= dalloc(&c->UW8, (size_t)c->Npad / 8 * vt * 16 * sizeof(uint4));
         r |= dalloc(&c->smax, (size_t)c->nsplit * c->kp * sizeof(unsigned));
         r |= dalloc(&c->Tpart, (size_t)c->nseg * B * c->kp * sizeof(float));
         r |= dalloc(&c->Apart, (size_t)c->nsplit * c->Mpad * c->kp * sizeof(float));
@@ -734,7 +735,7 @@ static int pass_prologue(dpca_ctx *c, const double *Qin)
 {
     if (c->p.precision != DPCA_MIXED) return 0;
     Stage s(c, T_MM);
-    DPCA_CHECK(launch_q8(c->st, Qin, c->Mpad, c->kp, c->seg_bytes, c->qmax, c->Q8));
+    DPCA_CHECK(launch_q8(c->st, Qin, c->Mpad, c->k, c->kp, c->seg_bytes, c->qmax, c->Q8));
     return 0;
 }
 
@@ -761,7 +762,7 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     {
         Stage s(c, T_MM_A);
         DPCA_CHECK(launch_za_i8(c->st, blk, rows, c->ldp, c->Q8, c->qmax, c->mu + r0,
-                                c->dsc + r0, c->kp, c->st_tiles, c->seg_bytes, c->nseg,
+                                c->dsc + r0, c->k, c->st_tiles, c->seg_bytes, c->nseg,
                                 c->Tpart));
     }
     {
@@ -770,15 +771,15 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     }
     if (half) return 0;
     const int nsplit = (int)((rows + c->rps - 1) / c->rps);
-    uint4 *UW8 = c->UW8 + (r0 / 8) * 3 * c->kp;
+    uint4 *UW8 = c->UW8 + (r0 / 8) * col_tiles(c->k).vt() * 16;
     {
         Stage s(c, T_MM);
-        DPCA_CHECK(launch_uw8(c->st, T, rows, c->kp, c->mu + r0, c->dsc + r0, c->rps, c->smax,
-                              UW8));
+        DPCA_CHECK(launch_uw8(c->st, T, rows, c->k, c->kp, c->mu + r0, c->dsc + r0, c->rps,
+                              c->smax, UW8));
     }
     {
         Stage s(c, T_MM_AT);
-        DPCA_CHECK(launch_zt_i8(c->st, blk, rows, c->ldp, UW8, c->smax, c->kp, c->sa_tiles,
+        DPCA_CHECK(launch_zt_i8(c->st, blk, rows, c->ldp, UW8, c->smax, c->k, c->sa_tiles,
                                 c->rps, nsplit, c->Apart));
     }
     Stage s(c, T_MM);
@@ -1858,6 +1859,7 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     m->N = c->N;
     m->Npad = c->Npad;
     m->nsv = nsv;
+    m->k = c->k;
     m->kp = c->kp;
     int r = 0;
     const size_t n1 = (size_t)c->Npad * sizeof(double);
@@ -1866,7 +1868,7 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     r |= dalloc(&m->dsc, n1);
     r |= dalloc(&m->W, n1 * c->kp);
     const int64_t prps = proj_rps_mx(c->Npad);
-    const size_t uw8_bytes = (size_t)c->Npad / 8 * 3 * c->kp * sizeof(uint4);
+    const size_t uw8_bytes = (size_t)c->Npad / 8 * col_tiles(c->k).vt() * 16 * sizeof(uint4);
     const size_t smax_bytes = (size_t)((c->Npad + prps - 1) / prps) * c->kp * sizeof(unsigned);
     if (m->precision == DPCA_MIXED) {
         r |= dalloc(&m->UW8, uw8_bytes);
@@ -1906,7 +1908,7 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->mu, c->mu, n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->dsc, c->dsc, n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess && m->UW8)
-        e = launch_uw8(c->st, m->W, c->Npad, c->kp, m->mu, m->dsc, prps, m->smax, m->UW8);
+        e = launch_uw8(c->st, m->W, c->Npad, c->k, c->kp, m->mu, m->dsc, prps, m->smax, m->UW8);
     if (!r && e == hipSuccess) e = hipStreamSynchronize(c->st);
     collect_timers(c);
     (void)hipFree(inv);

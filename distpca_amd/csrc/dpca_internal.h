@@ -82,25 +82,46 @@ hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t ro
                         double *T);
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
                              double *out, int accumulate);
+// Column tiles of the int8 forms: the three slices of k right-hand columns
+// as 16-wide MFMA tiles.  F whole column tiles take 3 tiles each; the R
+// columns left over (R <= 10) share RT = ceil(3R/16) tiles; R > 10 is padded
+// to a whole tile.  kp() is the padded column count of the FP64 buffers.
+struct ColTiles {
+    int F, R, RT;
+    int vt() const { return 3 * F + RT; }
+    int kp() const { return 16 * (F + (RT > 0 ? 1 : 0)); }
+};
+inline ColTiles col_tiles(int k)
+{
+    ColTiles t{k / 16, k % 16, 0};
+    if (t.R > 10) {
+        t.F += 1;
+        t.R = 0;
+    }
+    t.RT = (3 * t.R + 15) / 16;
+    return t;
+}
 // Z^T T on int8 MFMA: U = d T, W = -d mu T as three int8 slices per (SNP
 // split of rps rows, column) with a power-of-two scale from smax.
-// UW8: rows/8 x 3 x kp 16-byte words; smax: ceil(rows/rps) x kp.
-hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
-                      const double *dsc, int64_t rps, unsigned *smax, void *UW8);
+// UW8: rows/8 x vt x 16 words of 16 bytes; smax: ceil(rows/rps) x kp.
+hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int kp,
+                      const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8);
 hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const void *UW8, const unsigned *smax, int kp, int sa_tiles,
+                        const void *UW8, const unsigned *smax, int k, int sa_tiles,
                         int64_t rows_per_split, int nsplit, float *part);
 // Z Q on int8 MFMA: codes as exact h / m rows, Q as three int8 slices per
 // (sample segment of 4*seg_bytes samples, column) with a power-of-two scale
-// from qmax.  Q8: Mpad/16 x 3 x kp 16-byte words; qmax: nseg x kp.
-hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int kp, int64_t seg_bytes,
-                     unsigned *qmax, void *Q8);
+// from qmax.  Q8: Mpad/16 x vt x 16 words of 16 bytes; qmax: nseg x kp.
+hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int kp,
+                     int64_t seg_bytes, unsigned *qmax, void *Q8);
 hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                         const void *Q8, const unsigned *qmax, const double *mu,
-                        const double *dsc, int kp, int st_tiles, int64_t seg_bytes, int nseg,
+                        const double *dsc, int k, int st_tiles, int64_t seg_bytes, int nseg,
                         float *Tpart);
-// sample tiles per wave of k_zt_i8 for kp columns and a batch of ldb bytes
-inline int zt_i8_tiles(int kp, int64_t ldb) { return (kp <= 48 && ldb % 128 == 0) ? 4 : 2; }
+// tiles per wave (sample tiles of k_zt_i8 for a batch of ldb bytes, SNP
+// tiles of k_za_i8) that keep the accumulators in registers
+inline int zt_i8_tiles(int k, int64_t ldb) { return (col_tiles(k).vt() <= 9 && ldb % 128 == 0) ? 4 : 2; }
+inline int za_i8_tiles(int k) { return col_tiles(k).vt() <= 9 ? 2 : 1; }
 
 // Tall-skinny helpers, FP64.  X, Y: m x kp row-major.
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,

@@ -12,11 +12,6 @@
 //   Z^T T        two int8 K slots per SNP (h and m) against three slices of
 //                U = d T and W = -d mu T (scale per SNP split); splits
 //                summed in FP64.
-//
-// MFMA operand layout (16x16x64, checked on the device): lane l = 16g + r
-// holds A[r][16g..16g+15] and B[16g..16g+15][r] as bytes and receives
-// D[4g + q][r].  Which sample (or SNP) occupies which K slot only has to
-// agree between A and B.
 #include "dpca_internal.h"
 #include "k_common.h"
 
@@ -69,24 +64,26 @@ hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int6
     return hipGetLastError();
 }
 
-// ====================================================== Z^T T on int8 MFMA ==
-// Z^T T = sum_s [h_si U_s + m_si W_s] on v_mfma_i32_16x16x64_i8:
-// the codes give h (0/1/2) and m (0/1) as exact int8 K slots, and U, W are
-// split into three int8 slices of a fixed-point form with one power-of-two
-// scale per (SNP split, column):  U = s (q0 + q1 2^-7 + q2 2^-14) + e with
-// |e| <= s 2^-15 and s >= max(|U|, |W|) / 127 over the split.  Within a split
-// the slice products accumulate exactly in int32 (|sum| <= 381 per SNP), and
-// the three slice sums are combined in FP64 at the end of the split.
+// ============================================== int8 slices and their tiles ==
+// An FP64 operand x (Q, or U = d T and W = -d mu T) enters the MFMA as three
+// int8 slices of a fixed-point form with one power-of-two scale s per column
+// and group of rows (one workgroup's accumulation range):
+// x = s (q0 + q1 2^-7 + q2 2^-14) + e, |e| <= s 2^-15, s >= max|x| / 127.
+// The slice products sum exactly in int32; the slices are combined in FP64
+// when the group ends.
 //
-// MFMA operand layout (16x16x64, checked on the device): lane l = 16g + r
-// holds A[r][16g..16g+15] and B[16g..16g+15][r] as bytes and receives
-// D[4g + q][r].  K step t covers 32 SNPs; lane group g takes SNPs
-// 32t + 8g .. +7 with K bytes [h(0..3), h(4..7), m(0..3), m(4..7)].
+// Column tiles.  The three slices of the k right-hand columns are laid out
+// as 16-wide MFMA B tiles ("virtual tiles", VT = 3F + RT of them): columns
+// 16c..16c+15, c < F, take three tiles each (tile 3c + j holds slice j in
+// lane c%16); the R = k - 16F columns left over (R <= 10) share RT =
+// ceil(3R / 16) tiles: virtual column w = jR + x holds slice j of column
+// 16F + x, in tile 3F + w/16, lane w%16.  So k = 40 takes 8 tiles, not 9,
+// and k = 20 four, not six.  k with R > 10 is padded to whole tiles.
 //
-// UW8 layout: per 8-SNP group G, per slice j, per column c one 16-byte word
-// {U_j(8G..8G+3), U_j(8G+4..+7), W_j(8G..+3), W_j(8G+4..+7)} at
-// UW8[(G * 3 + j) * kp + c]; smax[split * kp + c] is the column's max of
-// |U|, |W| over the split, rounded up to float.
+// MFMA operand layout (v_mfma_i32_16x16x64_i8, checked on the device): lane
+// l = 16g + r holds A[r][16g..16g+15] and B[16g..16g+15][r] as bytes and
+// receives D[4g + q][r].  Which sample (or SNP) occupies which K byte only
+// has to agree between A and B.
 
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 
@@ -105,16 +102,64 @@ __device__ __forceinline__ double slice_scale(float mx)
 }
 
 // |x| <= 127 s: three int8 slices of x / s, each residual scaled by 2^7.
-__device__ __forceinline__ void slices3(double x, double inv_s, int &q0, int &q1, int &q2)
+__device__ __forceinline__ void slices3(double x, double inv_s, int q[3])
 {
     double v = x * inv_s;              // exact: inv_s is a power of two
     const double a = rint(v);
     v = (v - a) * 128.0;               // exact, |v| <= 64
     const double b = rint(v);
     v = (v - b) * 128.0;
-    q0 = (int)a;
-    q1 = (int)b;
-    q2 = (int)rint(v);
+    q[0] = (int)a;
+    q[1] = (int)b;
+    q[2] = (int)rint(v);
+}
+
+// Slice j and column of virtual column (tile v, lane r); column -1 = empty.
+__device__ __forceinline__ void vcol(int v, int r, int F, int R, int &j, int &col)
+{
+    if (v < 3 * F) {
+        j = v % 3;
+        col = (v / 3) * 16 + r;
+        return;
+    }
+    const int w = (v - 3 * F) * 16 + r;
+    j = R ? w / R : 3;
+    col = (j < 3) ? 16 * F + w % R : -1;
+}
+
+// Value of slice j of column 16F + x for this lane's D rows: the remainder
+// tiles' accumulators are spread over lanes 16g + (jR + x) % 16.
+template <int F, int RT>
+__device__ __forceinline__ int rem_slice(const i32x4 (&acc)[3 * F + RT], int q, int j, int x,
+                                         int R, int g)
+{
+    const int w = j * R + x;
+    const int src = 16 * g + (w & 15);
+    int val = 0;
+#pragma unroll
+    for (int u = 0; u < RT; u++) {
+        const int y = __shfl(acc[3 * F + u][q], src, 64);
+        if ((w >> 4) == u) val = y;
+    }
+    return val;
+}
+
+// Combined FP64 value sum_j 2^-7j slice_j of output column 16 oc + r for
+// D row 4g + q (zero for padding columns).
+template <int F, int RT>
+__device__ __forceinline__ double combine(const i32x4 (&acc)[3 * F + RT], int oc, int q, int r,
+                                          int R, int g)
+{
+    if (oc < F)
+        return (double)acc[3 * oc][q] + ldexp((double)acc[3 * oc + 1][q], -7) +
+               ldexp((double)acc[3 * oc + 2][q], -14);
+    // remainder columns: every lane joins the shuffles, lanes r >= R drop out
+    const int x = r < R ? r : 0;
+    const int s0 = rem_slice<F, RT>(acc, q, 0, x, R, g);
+    const int s1 = rem_slice<F, RT>(acc, q, 1, x, R, g);
+    const int s2 = rem_slice<F, RT>(acc, q, 2, x, R, g);
+    if (r >= R) return 0.0;
+    return (double)s0 + ldexp((double)s1, -7) + ldexp((double)s2, -14);
 }
 
 // smax[split * kp + c] = max over rows of the split of f_s |T_sc|, with
@@ -148,51 +193,61 @@ __global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
     }
 }
 
-// UW8 for rows [0, rows) of T (U = d T, W = -d mu T), split rps rows.
-__global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp,
+// ====================================================== Z^T T on int8 MFMA ==
+// Z^T T = sum_s [h_si U_s + m_si W_s]: the codes give h (0/1/2) and m (0/1)
+// as exact int8 K slots; U and W share one scale per (SNP split, column) so
+// both slots feed one accumulator (|sum| <= 381 per SNP).  K step t covers 32
+// SNPs; lane group g takes SNPs 32t + 8g .. +7 with K bytes
+// [h(0..3), h(4..7), m(0..3), m(4..7)].
+//
+// UW8 layout: per 8-SNP group G, per virtual tile v, per lane r one 16-byte
+// word {U_j(8G..+3), U_j(8G+4..+7), W_j(8G..+3), W_j(8G+4..+7)} of the slice
+// and column of (v, r), at UW8[(G * VT + v) * 16 + r].
+
+__global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F, int R, int VT,
                       const double *__restrict__ mu, const double *__restrict__ dsc, int64_t rps,
                       const unsigned *__restrict__ smax, u32x4 *__restrict__ UW8)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows / 8 * kp) return;
-    const int64_t G = idx / kp;
-    const int c = (int)(idx % kp);
-    const int64_t split = (G * 8) / rps;
-    const double s = slice_scale(__uint_as_float(smax[split * kp + c]));
-    const double inv = 1.0 / s;
-    uint32_t w[3][4] = {};
+    if (idx >= rows / 8 * VT * 16) return;
+    const int64_t G = idx / (VT * 16);
+    const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
+    int j, c;
+    vcol(v, r, F, R, j, c);
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (c >= 0) {
+        const int64_t split = (G * 8) / rps;
+        const double inv = 1.0 / slice_scale(__uint_as_float(smax[split * kp + c]));
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int64_t row = G * 8 + e;
-        const double t = T[row * kp + c];
-        int u0, u1, u2, v0, v1, v2;
-        slices3(dsc[row] * t, inv, u0, u1, u2);
-        slices3(-(dsc[row] * mu[row]) * t, inv, v0, v1, v2);
-        const int sh = 8 * (e & 3), wu = e >> 2, ww = 2 + (e >> 2);
-        w[0][wu] |= (uint32_t)(u0 & 0xFF) << sh;
-        w[1][wu] |= (uint32_t)(u1 & 0xFF) << sh;
-        w[2][wu] |= (uint32_t)(u2 & 0xFF) << sh;
-        w[0][ww] |= (uint32_t)(v0 & 0xFF) << sh;
-        w[1][ww] |= (uint32_t)(v1 & 0xFF) << sh;
-        w[2][ww] |= (uint32_t)(v2 & 0xFF) << sh;
+        for (int e = 0; e < 8; e++) {
+            const int64_t row = G * 8 + e;
+            const double t = T[row * kp + c];
+            int qu[3], qw[3];
+            slices3(dsc[row] * t, inv, qu);
+            slices3(-(dsc[row] * mu[row]) * t, inv, qw);
+            const int sh = 8 * (e & 3);
+            w[e >> 2] |= (uint32_t)(qu[j] & 0xFF) << sh;
+            w[2 + (e >> 2)] |= (uint32_t)(qw[j] & 0xFF) << sh;
+        }
     }
-#pragma unroll
-    for (int j = 0; j < 3; j++) UW8[(G * 3 + j) * kp + c] = u32x4{w[j][0], w[j][1], w[j][2], w[j][3]};
+    UW8[idx] = u32x4{w[0], w[1], w[2], w[3]};
 }
 
-hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int kp, const double *mu,
-                      const double *dsc, int64_t rps, unsigned *smax, void *UW8)
+hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int kp,
+                      const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8)
 {
-    if (rows % 128 || rps % 128 || rps < 128 || kp * 4 > 256) return hipErrorInvalidValue;
+    const ColTiles ct = col_tiles(k);
+    if (rows % 128 || rps % 128 || rps < 128 || kp * 4 > 256 || ct.kp() != kp)
+        return hipErrorInvalidValue;
     const int64_t nsplit = (rows + rps - 1) / rps;
     // a fill kernel, not a memset node: safe inside a captured graph
     hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(smax), nsplit * kp * 4, 0);
     if (e != hipSuccess) return e;
     dim3 g1((unsigned)((rps + 1023) / 1024), (unsigned)nsplit);
     k_split_max<<<g1, 4 * kp, 0, st>>>(T, rows, kp, mu, dsc, rps, smax);
-    const int64_t n = rows / 8 * kp;
-    k_uw8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, mu, dsc, rps, smax,
-                                                        reinterpret_cast<u32x4 *>(UW8));
+    const int64_t n = rows / 8 * ct.vt() * 16;
+    k_uw8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc,
+                                                        rps, smax, reinterpret_cast<u32x4 *>(UW8));
     return hipGetLastError();
 }
 
@@ -201,18 +256,19 @@ hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int kp, con
 // packed bytes staged in LDS transposed to [byte column][4-SNP group], so a
 // lane reads the codes of 4 consecutive SNPs for its sample with one dword
 // load, and the chunk's UW8 words copied to LDS.
-template <int SA, int CT>
+template <int SA, int F, int RT>
 __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ UW8,
                                                const unsigned *__restrict__ smax, int64_t rows,
-                                               int64_t rows_per_split, int64_t Mpad,
+                                               int64_t rows_per_split, int64_t Mpad, int R,
                                                float *__restrict__ part)
 {
+    constexpr int VT = 3 * F + RT;           // virtual column tiles
+    constexpr int CT = F + (RT > 0 ? 1 : 0); // output column tiles
     constexpr int KP = CT * 16;
-    constexpr int NS = 3;                  // int8 slices
     constexpr int TB = SA * 16 * 8 / 4;    // packed bytes per row in this tile
     constexpr int SC = 128;                // SNPs per chunk
-    constexpr int NUWT = (SC / 8) * NS * KP;  // UW8 words per chunk
+    constexpr int NUWT = (SC / 8) * VT * 16;  // UW8 words per chunk
     // two buffers: chunk i is multiplied out of one while chunk i+1 lands in
     // the other (one barrier per chunk).  pk rows are byte columns, their
     // dwords 4-SNP groups, stored at column sg ^ (row / 4) in rows of 36
@@ -232,13 +288,11 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
     if (send > rows) send = rows;
     const uint32_t HT = 0x02010000u, MT = 0x01010001u;  // h, m by 2-bit code
 
-    i32x4 acc[SA][CT][NS];
+    i32x4 acc[SA][VT];
 #pragma unroll
     for (int a = 0; a < SA; a++)
 #pragma unroll
-        for (int c = 0; c < CT; c++)
-#pragma unroll
-            for (int j = 0; j < NS; j++) acc[a][c][j] = i32x4{0, 0, 0, 0};
+        for (int v = 0; v < VT; v++) acc[a][v] = i32x4{0, 0, 0, 0};
 
     constexpr int NPK = (SC / 4) * (TB / 4) / 512;   // 4x4 byte blocks per thread
     constexpr int NUW = (NUWT + 511) / 512;          // UW8 words per thread
@@ -256,7 +310,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             _Pragma("unroll") for (int j = 0; j < 4; j++) rpk[u][j] =                      \
                 *reinterpret_cast<const uint32_t *>(p_ + j * ldp);                          \
         }                                                                                   \
-        const u32x4 *q_ = UW8 + ((c0) / 8) * NS * KP;                                       \
+        const u32x4 *q_ = UW8 + ((c0) / 8) * VT * 16;                                       \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
             const int it_ = tid + 512 * u;                                                  \
             if (it_ < NUWT) ruw[u] = q_[it_];                                               \
@@ -313,14 +367,10 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
                               __builtin_amdgcn_perm(0u, MT, c0), __builtin_amdgcn_perm(0u, MT, c1)};
             }
 #pragma unroll
-            for (int j = 0; j < NS; j++) {
-                u32x4 b[CT];
+            for (int v = 0; v < VT; v++) {
+                const u32x4 b = uw[cur][((4 * t + g) * VT + v) * 16 + r];
 #pragma unroll
-                for (int c = 0; c < CT; c++) b[c] = uw[cur][((4 * t + g) * NS + j) * KP + c * 16 + r];
-#pragma unroll
-                for (int a = 0; a < SA; a++)
-#pragma unroll
-                    for (int c = 0; c < CT; c++) acc[a][c][j] = mfma_i8(av[a], b[c], acc[a][c][j]);
+                for (int a = 0; a < SA; a++) acc[a][v] = mfma_i8(av[a], b, acc[a][v]);
             }
         }
         DPCA_ZT8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
@@ -330,103 +380,67 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
 #undef DPCA_ZT8_LOAD
     float *out = part + ((int64_t)blockIdx.y * Mpad + 4 * mb + wave * SA * 16) * KP;
 #pragma unroll
-    for (int c = 0; c < CT; c++) {
-        const double s = slice_scale(__uint_as_float(smax[blockIdx.y * KP + c * 16 + r]));
+    for (int oc = 0; oc < CT; oc++) {
+        const int col = oc * 16 + r;
+        const double s = slice_scale(__uint_as_float(smax[blockIdx.y * KP + col]));
 #pragma unroll
         for (int a = 0; a < SA; a++)
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const double v = (double)acc[a][c][0][q] + ldexp((double)acc[a][c][1][q], -7) +
-                                 ldexp((double)acc[a][c][2][q], -14);
-                out[(a * 16 + 4 * g + q) * KP + c * 16 + r] = (float)(s * v);
-            }
+            for (int q = 0; q < 4; q++)
+                out[(a * 16 + 4 * g + q) * KP + col] =
+                    (float)(s * combine<F, RT>(acc[a], oc, q, r, R, g));
     }
-}
-
-template <int SA>
-static hipError_t zt_i8_ct(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
-                           const u32x4 *UW8, const unsigned *smax, int kp, int64_t rows,
-                           int64_t rps, int64_t Mpad, float *part)
-{
-    switch (kp) {
-    case 16: k_zt_i8<SA, 1><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part); break;
-    case 32: k_zt_i8<SA, 2><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part); break;
-    case 48: k_zt_i8<SA, 3><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part); break;
-    case 64:
-        // 4 tiles x 4 column tiles x 3 slices would not fit the registers
-        if constexpr (SA <= 2) {
-            k_zt_i8<SA, 4><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, part);
-            break;
-        }
-        return hipErrorInvalidValue;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-// ldp must be a multiple of the tile's byte width (SA*32), rows and
-// rows_per_split multiples of 128, rows_per_split at most 2^22 (int32 sums).
-hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const void *UW8, const unsigned *smax, int kp, int sa_tiles,
-                        int64_t rows_per_split, int nsplit, float *part)
-{
-    const int64_t tb = (int64_t)sa_tiles * 32;
-    if (ldp % tb || rows % 128 || rows_per_split % 128 || rows_per_split > (1 << 22))
-        return hipErrorInvalidValue;
-    dim3 grid((unsigned)(ldp / tb), (unsigned)nsplit);
-    const int64_t Mpad = 4 * ldp;
-    const u32x4 *uw = reinterpret_cast<const u32x4 *>(UW8);
-    if (sa_tiles == 4) return zt_i8_ct<4>(st, grid, packed, ldp, uw, smax, kp, rows, rows_per_split, Mpad, part);
-    if (sa_tiles == 2) return zt_i8_ct<2>(st, grid, packed, ldp, uw, smax, kp, rows, rows_per_split, Mpad, part);
-    return hipErrorInvalidValue;
 }
 
 // ========================================================= Z Q on int8 MFMA ==
 // T = Z Q with T_s = d_s (sum_i h_si Q_i - mu_s sum_i m_si Q_i): the codes
-// give two exact int8 rows per SNP (h and m) and Q is split into three int8
-// slices with one power-of-two scale per (sample segment, column), so each
-// segment's sums are exact in int32 (|sum| <= 254 * 127 per 64 samples);
-// the slices, the two rows and the per-SNP factors are combined in FP64 at
-// the end of the segment and written as an FP32 partial of T.
+// give two exact int8 rows per SNP (h and m) against the slices of Q (one
+// scale per sample segment and column), so each segment's sums are exact in
+// int32 (|sum| <= 254 * 127 per 64 samples); the slices, the two rows and
+// the per-SNP factors are combined in FP64 at the end of the segment and
+// written as an FP32 partial of T.
 //
-// Q8 layout: per 16-sample group G, per slice j, per column c one 16-byte
-// word (byte e = sample 16G + e) at Q8[(G * 3 + j) * kp + c]; qmax[seg * kp
-// + c] the column's max |Q| over the segment.
-__global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int64_t seg_samples,
-                     const unsigned *__restrict__ qmax, u32x4 *__restrict__ Q8)
+// Q8 layout: per 16-sample group G, per virtual tile v, per lane r one
+// 16-byte word (byte e = sample 16G + e) of the slice and column of (v, r),
+// at Q8[(G * VT + v) * 16 + r]; qmax[seg * kp + c] the column's max |Q| over
+// the segment.
+__global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int F, int R, int VT,
+                     int64_t seg_samples, const unsigned *__restrict__ qmax,
+                     u32x4 *__restrict__ Q8)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= Mpad / 16 * kp) return;
-    const int64_t G = idx / kp;
-    const int c = (int)(idx % kp);
-    const int64_t seg = (G * 16) / seg_samples;
-    const double inv = 1.0 / slice_scale(__uint_as_float(qmax[seg * kp + c]));
-    uint32_t w[3][4] = {};
+    if (idx >= Mpad / 16 * VT * 16) return;
+    const int64_t G = idx / (VT * 16);
+    const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
+    int j, c;
+    vcol(v, r, F, R, j, c);
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (c >= 0) {
+        const int64_t seg = (G * 16) / seg_samples;
+        const double inv = 1.0 / slice_scale(__uint_as_float(qmax[seg * kp + c]));
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-        int q0, q1, q2;
-        slices3(Q[(G * 16 + e) * kp + c], inv, q0, q1, q2);
-        const int sh = 8 * (e & 3), wd = e >> 2;
-        w[0][wd] |= (uint32_t)(q0 & 0xFF) << sh;
-        w[1][wd] |= (uint32_t)(q1 & 0xFF) << sh;
-        w[2][wd] |= (uint32_t)(q2 & 0xFF) << sh;
+        for (int e = 0; e < 16; e++) {
+            int q[3];
+            slices3(Q[(G * 16 + e) * kp + c], inv, q);
+            w[e >> 2] |= (uint32_t)(q[j] & 0xFF) << (8 * (e & 3));
+        }
     }
-#pragma unroll
-    for (int j = 0; j < 3; j++) Q8[(G * 3 + j) * kp + c] = u32x4{w[j][0], w[j][1], w[j][2], w[j][3]};
+    Q8[idx] = u32x4{w[0], w[1], w[2], w[3]};
 }
 
-hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int kp, int64_t seg_bytes,
-                     unsigned *qmax, void *Q8)
+hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int kp,
+                     int64_t seg_bytes, unsigned *qmax, void *Q8)
 {
+    const ColTiles ct = col_tiles(k);
     const int64_t segs = 4 * seg_bytes;  // samples per segment
-    if (Mpad % 256 || segs % 256 || kp * 4 > 256) return hipErrorInvalidValue;
+    if (Mpad % 256 || segs % 256 || kp * 4 > 256 || ct.kp() != kp) return hipErrorInvalidValue;
     const int64_t nseg = (Mpad + segs - 1) / segs;
     hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(qmax), nseg * kp * 4, 0);
     if (e != hipSuccess) return e;
     dim3 g1((unsigned)((segs + 1023) / 1024), (unsigned)nseg);
     k_split_max<<<g1, 4 * kp, 0, st>>>(Q, Mpad, kp, nullptr, nullptr, segs, qmax);
-    const int64_t n = Mpad / 16 * kp;
-    k_q8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Q, Mpad, kp, segs, qmax,
+    const int64_t n = Mpad / 16 * ct.vt() * 16;
+    k_q8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Q, Mpad, kp, ct.F, ct.R, ct.vt(), segs, qmax,
                                                        reinterpret_cast<u32x4 *>(Q8));
     return hipGetLastError();
 }
@@ -435,18 +449,19 @@ hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int kp, int6
 // Per 256-sample chunk (64 packed bytes per row), lane group g owns the 64
 // samples of its 16 packed bytes; at K step t it supplies samples
 // 64g + 16t .. +15.  The chunk's Q8 words are staged in LDS.
-template <int ST, int CT>
+template <int ST, int F, int RT>
 __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ Q8,
                                                const unsigned *__restrict__ qmax,
                                                const double *__restrict__ mu,
                                                const double *__restrict__ dsc, int64_t seg_bytes,
-                                               int64_t Npad, float *__restrict__ Tpart)
+                                               int64_t Npad, int R, float *__restrict__ Tpart)
 {
+    constexpr int VT = 3 * F + RT;
+    constexpr int CT = F + (RT > 0 ? 1 : 0);
     constexpr int KP = CT * 16;
-    constexpr int NS = 3;
     constexpr int CB = 64;                      // packed bytes per chunk
-    constexpr int NQT = (CB / 4) * NS * KP;     // Q8 words per chunk
+    constexpr int NQT = (CB / 4) * VT * 16;     // Q8 words per chunk
     constexpr int NQ = (NQT + 511) / 512;
     __shared__ u32x4 qs[2][NQT];               // double buffer, one barrier per chunk
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -457,16 +472,14 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     if (b1 > ldp) b1 = ldp;
     const uint32_t HT = 0x02010000u, MT = 0x01010001u;  // h, m by 2-bit code
 
-    i32x4 acch[ST][CT][NS], accm[ST][CT][NS];
+    i32x4 acch[ST][VT], accm[ST][VT];
 #pragma unroll
     for (int s = 0; s < ST; s++)
 #pragma unroll
-        for (int c = 0; c < CT; c++)
-#pragma unroll
-            for (int j = 0; j < NS; j++) {
-                acch[s][c][j] = i32x4{0, 0, 0, 0};
-                accm[s][c][j] = i32x4{0, 0, 0, 0};
-            }
+        for (int v = 0; v < VT; v++) {
+            acch[s][v] = i32x4{0, 0, 0, 0};
+            accm[s][v] = i32x4{0, 0, 0, 0};
+        }
 
     // software pipeline: the next chunk's Q8 words and packed bytes are
     // loaded into registers while the current chunk is multiplied; the Q8
@@ -474,7 +487,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     u32x4 rq[NQ], rp[ST];
 #define DPCA_ZA8_LOAD(c0)                                                                   \
     do {                                                                                    \
-        const u32x4 *src_ = Q8 + ((c0) / 4) * NS * KP;                                      \
+        const u32x4 *src_ = Q8 + ((c0) / 4) * VT * 16;                                      \
         _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
             const int it_ = tid + 512 * u;                                                  \
             if (it_ < NQT) rq[u] = src_[it_];                                               \
@@ -505,28 +518,25 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
         }
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            u32x4 b[NS][CT];
-#pragma unroll
-            for (int j = 0; j < NS; j++)
-#pragma unroll
-                for (int c = 0; c < CT; c++) b[j][c] = qs[cur][((4 * g + t) * NS + j) * KP + c * 16 + r];
+            u32x4 ah[ST], am[ST];
 #pragma unroll
             for (int s = 0; s < ST; s++) {
                 const uint32_t w = pk[s][t];
-                u32x4 ah, am;
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     const uint32_t cw = spread4((w >> (8 * k)) & 0xFFu);
-                    ah[k] = __builtin_amdgcn_perm(0u, HT, cw);
-                    am[k] = __builtin_amdgcn_perm(0u, MT, cw);
+                    ah[s][k] = __builtin_amdgcn_perm(0u, HT, cw);
+                    am[s][k] = __builtin_amdgcn_perm(0u, MT, cw);
                 }
+            }
 #pragma unroll
-                for (int j = 0; j < NS; j++)
+            for (int v = 0; v < VT; v++) {
+                const u32x4 b = qs[cur][((4 * g + t) * VT + v) * 16 + r];
 #pragma unroll
-                    for (int c = 0; c < CT; c++) {
-                        acch[s][c][j] = mfma_i8(ah, b[j][c], acch[s][c][j]);
-                        accm[s][c][j] = mfma_i8(am, b[j][c], accm[s][c][j]);
-                    }
+                for (int s = 0; s < ST; s++) {
+                    acch[s][v] = mfma_i8(ah[s], b, acch[s][v]);
+                    accm[s][v] = mfma_i8(am[s], b, accm[s][v]);
+                }
             }
         }
         DPCA_ZA8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
@@ -536,57 +546,106 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 #undef DPCA_ZA8_LOAD
     float *out = Tpart + (int64_t)blockIdx.y * Npad * KP;
 #pragma unroll
-    for (int c = 0; c < CT; c++) {
-        const double sc = slice_scale(__uint_as_float(qmax[blockIdx.y * KP + c * 16 + r]));
+    for (int oc = 0; oc < CT; oc++) {
+        const int col = oc * 16 + r;
+        const double sc = slice_scale(__uint_as_float(qmax[blockIdx.y * KP + col]));
 #pragma unroll
         for (int s = 0; s < ST; s++)
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int64_t row = base + s * 16 + 4 * g + q;
-                const double h = (double)acch[s][c][0][q] + ldexp((double)acch[s][c][1][q], -7) +
-                                 ldexp((double)acch[s][c][2][q], -14);
-                const double m = (double)accm[s][c][0][q] + ldexp((double)accm[s][c][1][q], -7) +
-                                 ldexp((double)accm[s][c][2][q], -14);
-                out[row * KP + c * 16 + r] = (float)(dsc[row] * sc * (h - mu[row] * m));
+                const double h = combine<F, RT>(acch[s], oc, q, r, R, g);
+                const double m = combine<F, RT>(accm[s], oc, q, r, R, g);
+                out[row * KP + col] = (float)(dsc[row] * sc * (h - mu[row] * m));
             }
     }
 }
 
-template <int ST>
-static hipError_t za_i8_ct(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
-                           const u32x4 *Q8, const unsigned *qmax, const double *mu,
-                           const double *dsc, int kp, int64_t seg_bytes, int64_t rows,
-                           float *Tpart)
+// ------------------------------------------------------------- launchers --
+// Instantiated tile shapes: (F, RT) of col_tiles(k) for k = 1..64, with the
+// sample / SNP tiles per wave chosen so the accumulators fit the registers.
+#define DPCA_I8_SHAPES(X) \
+    X(1, 0) X(2, 0) X(3, 0) X(4, 0) X(0, 1) X(1, 1) X(2, 1) X(3, 1) X(0, 2) X(1, 2) X(2, 2) X(3, 2)
+
+template <int SA, int F, int RT>
+static hipError_t zt_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
+                        const u32x4 *UW8, const unsigned *smax, int64_t rows, int64_t rps,
+                        int64_t Mpad, int R, float *part)
 {
-    switch (kp) {
-    case 16: k_za_i8<ST, 1><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
-    case 32: k_za_i8<ST, 2><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
-    case 48: k_za_i8<ST, 3><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart); break;
-    case 64:
-        // 2 tiles x 4 column tiles x 3 slices x 2 rows would not fit the registers
-        if constexpr (ST <= 1) {
-            k_za_i8<ST, 4><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows, Tpart);
-            break;
-        }
-        return hipErrorInvalidValue;
-    default: return hipErrorInvalidValue;
+    if constexpr (SA * (3 * F + RT) > 36) {
+        return hipErrorInvalidValue;  // would not fit the registers
+    } else {
+        k_zt_i8<SA, F, RT><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, R, part);
+        return hipGetLastError();
     }
-    return hipGetLastError();
+}
+
+template <int ST, int F, int RT>
+static hipError_t za_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
+                        const u32x4 *Q8, const unsigned *qmax, const double *mu,
+                        const double *dsc, int64_t seg_bytes, int64_t rows, int R, float *Tpart)
+{
+    if constexpr (ST * (3 * F + RT) > 18) {
+        return hipErrorInvalidValue;  // would not fit the registers
+    } else {
+        k_za_i8<ST, F, RT><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows,
+                                                 R, Tpart);
+        return hipGetLastError();
+    }
+}
+
+// ldp must be a multiple of the tile's byte width (SA*32), rows and
+// rows_per_split multiples of 128, rows_per_split at most 2^22 (int32 sums).
+hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                        const void *UW8, const unsigned *smax, int k, int sa_tiles,
+                        int64_t rows_per_split, int nsplit, float *part)
+{
+    const ColTiles ct = col_tiles(k);
+    const int64_t tb = (int64_t)sa_tiles * 32;
+    if (ldp % tb || rows % 128 || rows_per_split % 128 || rows_per_split > (1 << 22))
+        return hipErrorInvalidValue;
+    dim3 grid((unsigned)(ldp / tb), (unsigned)nsplit);
+    const int64_t Mpad = 4 * ldp;
+    const u32x4 *uw = reinterpret_cast<const u32x4 *>(UW8);
+#define X(F_, RT_)                                                                          \
+    if (ct.F == F_ && ct.RT == RT_) {                                                        \
+        if (sa_tiles == 4)                                                                   \
+            return zt_go<4, F_, RT_>(st, grid, packed, ldp, uw, smax, rows, rows_per_split,  \
+                                     Mpad, ct.R, part);                                      \
+        if (sa_tiles == 2)                                                                   \
+            return zt_go<2, F_, RT_>(st, grid, packed, ldp, uw, smax, rows, rows_per_split,  \
+                                     Mpad, ct.R, part);                                      \
+        return hipErrorInvalidValue;                                                         \
+    }
+    DPCA_I8_SHAPES(X)
+#undef X
+    return hipErrorInvalidValue;
 }
 
 // rows a multiple of 8*16*st_tiles, seg_bytes of 64, segments of at most
 // 65536 samples (int32 sums).
 hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                         const void *Q8, const unsigned *qmax, const double *mu,
-                        const double *dsc, int kp, int st_tiles, int64_t seg_bytes, int nseg,
+                        const double *dsc, int k, int st_tiles, int64_t seg_bytes, int nseg,
                         float *Tpart)
 {
+    const ColTiles ct = col_tiles(k);
     const int64_t per = 8 * 16 * (int64_t)st_tiles;
     if (rows % per || seg_bytes % 64 || ldp % 64 || seg_bytes > 16384) return hipErrorInvalidValue;
     dim3 grid((unsigned)(rows / per), (unsigned)nseg);
     const u32x4 *q = reinterpret_cast<const u32x4 *>(Q8);
-    if (st_tiles == 2) return za_i8_ct<2>(st, grid, packed, ldp, q, qmax, mu, dsc, kp, seg_bytes, rows, Tpart);
-    if (st_tiles == 1) return za_i8_ct<1>(st, grid, packed, ldp, q, qmax, mu, dsc, kp, seg_bytes, rows, Tpart);
+#define X(F_, RT_)                                                                          \
+    if (ct.F == F_ && ct.RT == RT_) {                                                        \
+        if (st_tiles == 2)                                                                   \
+            return za_go<2, F_, RT_>(st, grid, packed, ldp, q, qmax, mu, dsc, seg_bytes,     \
+                                     rows, ct.R, Tpart);                                     \
+        if (st_tiles == 1)                                                                   \
+            return za_go<1, F_, RT_>(st, grid, packed, ldp, q, qmax, mu, dsc, seg_bytes,     \
+                                     rows, ct.R, Tpart);                                     \
+        return hipErrorInvalidValue;                                                         \
+    }
+    DPCA_I8_SHAPES(X)
+#undef X
     return hipErrorInvalidValue;
 }
 

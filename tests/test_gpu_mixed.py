@@ -35,6 +35,23 @@ def test_mixed_pass(M, N, k):
     assert np.array_equal(got, got2)  # fixed reduction order: bitwise repeatable
 
 
+@pytest.mark.parametrize("k", [1, 5, 10, 16, 26, 37, 44, 48, 53, 58, 64])
+def test_mixed_pass_column_tile_shapes(k):
+    # every packing of the three slices into column tiles: whole tiles only
+    # (16, 48, 64; 44 padded to 48), one shared remainder tile (1, 5, 37, 53)
+    # and two (10, 26, 58)
+    M, N = 300, 2000
+    print(f"[gpu] mixed pass tile shape k={k}", flush=True)
+    packed = _packed_random(M, N, seed=100 + k)
+    Q = np.linalg.qr(np.random.default_rng(k).standard_normal((M, k)))[0]
+    ref = O.OOCPass(packed, M, N, 512)(Q)
+    p = D.Params(nsv=min(k, 2), nrhs=k, bsize=512, precision=D.MIXED)
+    with D.PCA.from_packed(packed, M, N, p) as pca:
+        got = pca.pass_(Q)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 2e-5, (k, err)
+
+
 def test_mixed_c1(c1_golden):
     print("[gpu] mixed c1 run", flush=True)
     spec = D.gen_spec(1024, 50000, npop=3, fst=0.05, missing=0.01, seed=0)
