@@ -277,9 +277,14 @@ def main():
     dom_name = "k_za_i8" if dom == 1 else "k_zt_i8"
     achieved = flops_kernel / (ms[dom] * 1e-3)
     traffic = load_traffic(f"{a.workload}_{dom_name}")
-    # issued int8 MFMA work: 3 slices x 2 (h, m) x padded columns kp
-    kp = (nrhs + 15) // 16 * 16
-    issued = 2.0 * M * N * kp * 3 * 2
+    # issued int8 MFMA work: two code slots (h, m) x the 16-wide column tiles
+    # that hold the three slices of nrhs columns (the partial last column
+    # tile's slices share tiles: nrhs 40 -> 8 tiles)
+    F, R = divmod(nrhs, 16)
+    if R > 10:
+        F, R = F + 1, 0
+    tiles = 3 * F + (3 * R + 15) // 16
+    issued = 2.0 * M * N * 16 * tiles * 2
     roofline = {"bound": "mfma", "kernel": dom_name, "achieved": round(achieved / 1e12, 2),
                 "peak": PEAK_INT8 / 1e12, "unit": "TOP/s",
                 "frac": round(achieved / PEAK_INT8, 4),
