@@ -46,7 +46,7 @@ class Report(ctypes.Structure):
         ("t_gs", ctypes.c_double), ("t_rr", ctypes.c_double), ("t_other", ctypes.c_double),
         ("wall", ctypes.c_double), ("last_crit", ctypes.c_double),
         ("t_h2d", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
-        ("host_mode", ctypes.c_int),
+        ("host_mode", ctypes.c_int), ("qr_repaired", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -112,6 +112,74 @@ void print_statistics(const dpca_report &r, int threads, int rkb, bool fullsvd)
     }
 }
 
+int llc_kb()
+{
+    // get_llc_size_kb (src/utilities.cpp:101-126)
+    FILE *fp = fopen("/sys/devices/system/cpu/cpu0/cache/index3/size", "r");
+    if (!fp) return -1;
+    int kb = -1;
+    if (fscanf(fp, "%dK", &kb) != 1) kb = -1;
+    fclose(fp);
+    return kb;
+}
+
+// Memory, cache and block-size lines of src/distpca.cpp:219-306 for one rank.
+// p holds the flags as given; bsize is the block size the library resolved
+// from them (the reference's rows_fetched).
+void print_sizing(const dpca_params &p, int M, long long N, int bsize, int rkb)
+{
+    const double mem = p.memory == 0.0 ? rkb / 1e6 / 10.0 : p.memory;
+    printf("\n%sMemory per rank: %g GB\n", stamp().c_str(), mem);
+    const int llc = llc_kb();
+    if (llc > 0) printf("\n%sDetected LLC size: %d KB (%d MB)\n", stamp().c_str(), llc, llc / 1024);
+    if (p.bsize > 0 && p.bsize > N)
+        printf("\n%sWarning: Specified block size (%d) exceeds local SNPs (%lld). Adjusting to "
+               "local_N.\n",
+               stamp().c_str(), p.bsize, N);
+    printf("\n%sBlock size (SNPs per block): %d\n", stamp().c_str(), bsize);
+    // the auxiliary estimate uses NRHS as given, before its clamps
+    const double block_gb = 2.0 * bsize * (double)M * 8.0 / 1e9;
+    const double aux_gb = (double)M * p.nrhs * 8.0 * 3 / 1e9;
+    printf("\n%sEstimated memory per rank: %g GB (Block buffers: %g GB, Auxiliary arrays: %g GB)\n",
+           stamp().c_str(), block_gb + aux_gb, block_gb, aux_gb);
+    printf("\n%sBlock size (in MB): %g (typical LLC sizes: 16-64 MB per socket)\n",
+           stamp().c_str(), (double)bsize * (double)M * 8.0 / (1024.0 * 1024.0));
+}
+
+// The NSV / NRHS adjustments of src/distpca.cpp:325-352, replayed on the
+// flags as given to print the reference's messages; the library applies the
+// same rules when it resolves the parameters.
+void print_clamps(const dpca_params &p, int M, long long N)
+{
+    const int mn = (int)(M < N ? M : N);
+    int s = p.nsv, r = p.nrhs;
+    auto two_s = [&] { return 2 * s < mn ? 2 * s : mn; };
+    if (s > mn) {
+        s = mn;
+        printf("\n%sThe value of NSV given was larger than min(M, N). Adjusting to NSV = "
+               "min(M,N)...\n",
+               stamp().c_str());
+    }
+    if (r <= 0) {
+        printf("\n%sNRHS was either zero or negative. Adjusting to NRHS = min(2xNSV, "
+               "min(M,N))...\n",
+               stamp().c_str());
+        r = two_s();
+    }
+    if (r > mn) {
+        r = mn;
+        printf("\n%sThe value of NRHS given was larger than min(M,N). Adjusting to NRHS = "
+               "min(M,N)...\n",
+               stamp().c_str());
+    }
+    if (r < s) {
+        printf("\n%sNRHS can not be smaller than NSV. Adjusting to NRHS = "
+               "min(2xNSV,min(M,N))...\n",
+               stamp().c_str());
+        r = two_s();
+    }
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -170,10 +238,14 @@ int main(int argc, char **argv)
     const std::string bfile(fname);
     printf("%sReading .fam file: %s.fam\n", stamp().c_str(), bfile.c_str());
     dpca_ctx *ctx = nullptr;
+    struct timespec o0, o1;
+    clock_gettime(CLOCK_MONOTONIC, &o0);
     if (dpca_open(bfile.c_str(), &p, &ctx) != 0) {
         printf("\n%s\n", dpca_last_error());
         return 1;
     }
+    clock_gettime(CLOCK_MONOTONIC, &o1);
+    const double t_open = (o1.tv_sec - o0.tv_sec) + 1e-9 * (o1.tv_nsec - o0.tv_nsec);
     int M, nsv, nrhs, bsize;
     long long N;
     dpca_get_dims(ctx, &M, &N, &nsv, &nrhs, &bsize);
@@ -186,7 +258,14 @@ int main(int argc, char **argv)
     printf("\n%sDataset partitioning across MPI ranks:\n\n", stamp().c_str());
     printf("Rank 0: SNPs [0, %lld) = %lld SNPs\n", N, N);
     printf("\n%sSize of RAM (in KB): %d\n", stamp().c_str(), rkb);
-    printf("\n%sBlock size (SNPs per block): %d\n", stamp().c_str(), bsize);
+    print_sizing(p, M, N, bsize, rkb);
+    print_clamps(p, M, N);
+    if (bsize == N) {
+        // src/distpca.cpp:366-377; the library reads the .bed inside
+        // dpca_open, so the time is that call's
+        printf("\n%sReading .bed file (distributed across ranks)...\n", stamp().c_str());
+        printf("\n%sFinished reading local data. Time: %g sec\n", stamp().c_str(), t_open);
+    }
     printf("\n%sPrecision: %s on HIP device %d\n", stamp().c_str(),
            p.precision == DPCA_MIXED ? "mixed" : "fp64", p.device);
     fflush(stdout);

@@ -184,6 +184,9 @@ struct dpca_ctx {
     bool stats_done = false;
     double t_h2d = 0, bytes_h2d = 0, t_stream_pass = 0;
     int *flags = nullptr;             // [0] shifted used, [1] QR failure, [2] Jacobi
+    double *cinfo = nullptr;          // dependent columns of the last mode-0 factor
+    bool robust = false;              // orthonormalise completes rank-deficient bases
+    int qr_repairs = 0;               // bases completed in the last run
     IterState *its = nullptr;
     std::vector<double> crit_hist;
     // timers
@@ -263,7 +266,7 @@ static void free_ctx(dpca_ctx *c)
     (void)hipSetDevice(c->dev);
     void *ptrs[] = {c->packed, c->zval, c->mu, c->dsc, c->ssum, c->sctr, c->Q, c->ARHS,
                     c->RHS2, c->RHS2old, c->tmp, c->T, c->part, c->gpart, c->G, c->Rinv,
-                    c->w, c->X, c->dots, c->flags, c->its, c->UW8, c->Q8, c->qmax, c->smax, c->Tpart, c->Apart};
+                    c->w, c->X, c->dots, c->flags, c->cinfo, c->its, c->UW8, c->Q8, c->qmax, c->smax, c->Tpart, c->Apart};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (int b = 0; b < 2; b++) {
@@ -406,6 +409,7 @@ static int alloc_work(dpca_ctx *c)
     r |= dalloc(&c->X, 64 * 64 * sizeof(double));
     r |= dalloc(&c->dots, 64 * sizeof(double));
     r |= dalloc(&c->flags, 16 * sizeof(int));
+    r |= dalloc(&c->cinfo, (kMaxKp + 1) * sizeof(double));
     r |= dalloc(&c->its, sizeof(IterState));
     if (r) return -3;
     if (g_dry) return 0;
@@ -447,12 +451,7 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
     c->dev = p->device;
     c->M = M;
     c->N = N;
-    hipError_t e = hipSetDevice(c->dev);
-    if (e != hipSuccess) {
-        set_error("hipSetDevice(%d): %s", c->dev, hipGetErrorString(e));
-        delete c;
-        return -2;
-    }
+    // parameter errors are reported before any device call
     int r = resolve_params(c, p);
     if (r) {
         delete c;
@@ -462,6 +461,12 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
         set_error("unknown precision %d", c->p.precision);
         delete c;
         return -1;
+    }
+    hipError_t e = hipSetDevice(c->dev);
+    if (e != hipSuccess) {
+        set_error("hipSetDevice(%d): %s", c->dev, hipGetErrorString(e));
+        delete c;
+        return -2;
     }
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
@@ -841,18 +846,40 @@ static int half_pass(dpca_ctx *c, const double *Qin)
 }
 
 // CholQR2 (+ shifted fallback) of A (Mpad x kp) into c->Q; uses c->tmp.
-static int orthonormalise(dpca_ctx *c, const double *A)
+// In robust mode a basis that spans fewer than k dimensions (the shifted
+// fallback still meets a non-positive pivot) has its dependent columns
+// replaced in place by k_replace_cols and is factored again, so the result
+// is an orthonormal completion of its span, as dgeqrf + dorgqr would give;
+// this costs one host synchronisation per call, so it is only switched on for
+// a context whose plain run met such a basis.
+static int orthonormalise(dpca_ctx *c, double *A)
 {
     Stage s(c, T_GS);
     const double u = ldexp(1.0, -53);
     const double shift = 11.0 * ((double)c->M * c->k + (double)c->k * (c->k + 1)) * u;
-    const double *src = A;
-    double *dst[3] = {c->tmp, c->Q, c->tmp};
-    for (int mode = 0; mode < 3; mode++) {
-        DPCA_CHECK(launch_gram(c->st, src, src, c->Mpad, c->kp, c->gpart, c->ngparts, c->G));
-        DPCA_CHECK(launch_chol_inv(c->st, c->G, c->k, c->kp, shift, mode, c->Rinv, c->flags));
-        DPCA_CHECK(launch_apply(c->st, src, c->Rinv, c->Mpad, c->kp, dst[mode]));
-        src = dst[mode];
+    for (int attempt = 0;; attempt++) {
+        const double *src = A;
+        double *dst[3] = {c->tmp, c->Q, c->tmp};
+        for (int mode = 0; mode < 3; mode++) {
+            DPCA_CHECK(launch_gram(c->st, src, src, c->Mpad, c->kp, c->gpart, c->ngparts, c->G));
+            DPCA_CHECK(launch_chol_inv(c->st, c->G, c->k, c->kp, shift, mode, c->Rinv, c->flags,
+                                       c->robust ? c->cinfo : nullptr));
+            DPCA_CHECK(launch_apply(c->st, src, c->Rinv, c->Mpad, c->kp, dst[mode]));
+            src = dst[mode];
+        }
+        if (!c->robust) break;
+        int failed = 0;
+        DPCA_CHECK(hipMemcpyAsync(&failed, c->flags + 1, sizeof(int), hipMemcpyDeviceToHost,
+                                  c->st));
+        DPCA_CHECK(hipStreamSynchronize(c->st));
+        if (!failed) break;
+        if (attempt == 1) {
+            set_error("CholeskyQR failed after completing a rank-deficient basis");
+            return -4;
+        }
+        DPCA_CHECK(launch_replace_cols(c->st, A, c->M, c->Mpad, c->k, c->kp, c->cinfo));
+        DPCA_CHECK(hipMemsetAsync(c->flags + 1, 0, sizeof(int), c->st));
+        c->qr_repairs++;
     }
     std::swap(c->Q, c->tmp);  // the third pass landed in tmp
     return 0;
@@ -877,13 +904,20 @@ static int copy_out_rhs(dpca_ctx *c, const double *dev, double *host)
     return 0;
 }
 
-static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
+// Returned by run_once when a plain run met a rank-deficient basis; run()
+// then repeats it in robust mode.
+constexpr int kRerunRobust = 1;
+
+static int run_once(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep, double t0,
+                    bool fresh)
 {
-    const double t0 = now_s();
     const dpca_params &p = c->p;
-    for (int i = 0; i < T_NKIND; i++) c->t[i] = 0;
-    c->passes = 0;
-    c->bytes_h2d = 0;
+    if (fresh) {
+        for (int i = 0; i < T_NKIND; i++) c->t[i] = 0;
+        c->passes = 0;
+        c->bytes_h2d = 0;
+        c->qr_repairs = 0;
+    }
     c->crit_hist.clear();
     std::vector<double> om;
     double *rhs_in = rhs;
@@ -972,6 +1006,8 @@ static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
     const int total_its = (ii < p.miter) ? ii + 1 : ii;
     int flags[16];
     DPCA_CHECK(hipMemcpyAsync(flags, c->flags, sizeof flags, hipMemcpyDeviceToHost, c->st));
+    DPCA_CHECK(hipStreamSynchronize(c->st));
+    if (flags[1] && !c->robust) return kRerunRobust;  // rhs is still the caller's input
     if (rhs && (r = copy_out_rhs(c, c->RHS2, rhs))) return r;
     DPCA_CHECK(hipStreamSynchronize(c->st));
     collect_timers(c);
@@ -1012,8 +1048,21 @@ static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
         rep->t_h2d = c->t[T_H2D];
         rep->h2d_bytes = c->bytes_h2d;
         rep->host_mode = c->streamed ? c->hmode : -1;
+        rep->qr_repaired = c->qr_repairs;
     }
     return 0;
+}
+
+static int run(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep)
+{
+    const double t0 = now_s();
+    int r = run_once(c, rhs, sigma, rep, t0, true);
+    if (r == kRerunRobust) {
+        // the time of the plain attempt stays in the stage timers
+        c->robust = true;
+        r = run_once(c, rhs, sigma, rep, t0, false);
+    }
+    return r;
 }
 
 // ---------------------------------------------------------------- PLINK IO --
@@ -1540,7 +1589,9 @@ int dpca_orthonormalize(const double *A, long long m, int k, double *Q, int *shi
     const size_t bytes = (size_t)m * c.kp * 8;
     int r = dalloc(&c.tmp, bytes) | dalloc(&c.Q, bytes) | dalloc(&c.ARHS, bytes) |
             dalloc(&c.gpart, (size_t)c.ngparts * 64 * 64 * 8) | dalloc(&c.G, 64 * 64 * 8) |
-            dalloc(&c.Rinv, 64 * 64 * 8) | dalloc(&c.flags, 64);
+            dalloc(&c.Rinv, 64 * 64 * 8) | dalloc(&c.flags, 64) |
+            dalloc(&c.cinfo, (kMaxKp + 1) * 8);
+    c.robust = true;  // complete a rank-deficient A, as dgeqrf + dorgqr do
     int fl[16] = {0};
     if (!r) {
         hipError_t e = hipMemsetAsync(c.ARHS, 0, bytes, c.st);
@@ -1561,9 +1612,9 @@ int dpca_orthonormalize(const double *A, long long m, int k, double *Q, int *shi
     }
     collect_timers(&c);
     for (void *p : {(void *)c.tmp, (void *)c.Q, (void *)c.ARHS, (void *)c.gpart, (void *)c.G,
-                    (void *)c.Rinv, (void *)c.flags})
+                    (void *)c.Rinv, (void *)c.flags, (void *)c.cinfo})
         if (p) (void)hipFree(p);
-    c.tmp = c.Q = c.ARHS = c.gpart = c.G = c.Rinv = nullptr;
+    c.tmp = c.Q = c.ARHS = c.gpart = c.G = c.Rinv = c.cinfo = nullptr;
     c.flags = nullptr;
     for (auto e : c.free_ev) (void)hipEventDestroy(e);
     c.free_ev.clear();

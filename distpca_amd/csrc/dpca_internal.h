@@ -27,6 +27,7 @@ void set_error(const char *fmt, ...);
 constexpr int kRowAlign = 1024;   // SNP rows padded to this multiple
 constexpr int kLdpAlign = 256;    // bytes per padded row: multiple of this
 constexpr uint8_t kPadByte = 0x55;
+constexpr int kMaxKp = 64;         // right-hand sides, padded (k <= 64)
 
 inline int64_t round_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 
@@ -127,7 +128,9 @@ inline int za_i8_tiles(int k) { return col_tiles(k).vt() <= 9 ? 2 : 1; }
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
                        double *part, int nparts, double *G);
 hipError_t launch_chol_inv(hipStream_t st, const double *G, int k, int kp, double shift_mult,
-                           int mode, double *Rinv, int *flags);
+                           int mode, double *Rinv, int *flags, double *cinfo);
+hipError_t launch_replace_cols(hipStream_t st, double *A, int M, int64_t Mpad, int k, int kp,
+                               const double *cinfo);
 hipError_t launch_apply(hipStream_t st, const double *X, const double *W, int64_t m, int kp,
                         double *Y);
 hipError_t launch_syev(hipStream_t st, const double *B, int k, int kp, double *w, double *V,

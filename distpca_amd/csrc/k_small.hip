@@ -8,8 +8,6 @@
 
 namespace dpca {
 
-constexpr int kMaxKp = 64;
-
 // ------------------------------------------------------------------- Gram --
 // part[p][a*kp + b] = sum_{i in rows of p} X[i][a] * Y[i][b]; 32-row tiles of
 // X and Y are staged in LDS; thread t owns entries t, t+256, ...
@@ -96,19 +94,27 @@ hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t
 // Right-looking Cholesky of the leading k x k block of A (upper triangle).
 // A pivot that is not positive, or that has lost all but ~1e-14 of its
 // column's diagonal (the column lies within ~1e-7 rad of the span of the
-// previous ones, beyond what CholQR2 can orthonormalise), marks the factor bad.
-__device__ int chol_factor(double (*A)[kMaxKp + 1], const double *diag0, int k, int *bad)
+// previous ones, beyond what CholQR2 can orthonormalise), marks the factor bad
+// and sets colbad[j]; that column's row of R is then zeroed beyond the
+// diagonal, so the later pivots are those of the basis without it and colbad
+// ends up naming a set of columns whose removal leaves the rest independent.
+__device__ int chol_factor(double (*A)[kMaxKp + 1], const double *diag0, int k, int *bad,
+                           int *colbad)
 {
     const int tid = threadIdx.x;
+    __shared__ int drop;
     for (int j = 0; j < k; j++) {
         if (tid == 0) {
             const double d = A[j][j];
-            if (!(d > 0.0) || !(d > 1e-14 * diag0[j])) *bad = 1;
-            A[j][j] = sqrt(d > 0.0 ? d : 1.0);
+            drop = !(d > 0.0) || !(d > 1e-14 * diag0[j]);
+            if (drop) *bad = 1;
+            colbad[j] = drop;
+            A[j][j] = drop ? 1.0 : sqrt(d);
         }
         __syncthreads();
         const double piv = A[j][j];
-        for (int c = j + 1 + tid; c < k; c += 256) A[j][c] /= piv;
+        const bool dj = drop;
+        for (int c = j + 1 + tid; c < k; c += 256) A[j][c] = dj ? 0.0 : A[j][c] / piv;
         __syncthreads();
         const int n = k - j - 1;
         for (int e = tid; e < n * n; e += 256) {
@@ -121,14 +127,19 @@ __device__ int chol_factor(double (*A)[kMaxKp + 1], const double *diag0, int k, 
 }
 
 // mode 2 also runs a real third pass when the Gram of the CholQR2 result is
-// not the identity to 1e-12 (flags[3] records it).
+// not the identity to 1e-12 (flags[3] records it).  When cinfo is not NULL,
+// mode 0 records the unshifted factor's dependent columns there (cinfo[j] =
+// 1.0 for a dropped column j, else 0.0) and the largest diagonal of G in
+// cinfo[kMaxKp], for k_replace_cols.
 __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, int k,
                                                   int kp, double shift_mult, int mode,
-                                                  double *__restrict__ Rinv, int *flags)
+                                                  double *__restrict__ Rinv, int *flags,
+                                                  double *__restrict__ cinfo)
 {
     __shared__ double A[kMaxKp][kMaxKp + 1];
     __shared__ double col[kMaxKp];
     __shared__ double diag0[kMaxKp];
+    __shared__ int colbad[kMaxKp];
     __shared__ int bad, skip;
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -155,7 +166,15 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
             Rinv[e] = (e / kp == e % kp && e / kp < k) ? 1.0 : 0.0;
         return;
     }
-    chol_factor(A, diag0, k, &bad);
+    chol_factor(A, diag0, k, &bad, colbad);
+    if (mode == 0 && cinfo) {
+        if (tid < k) cinfo[tid] = colbad[tid] ? 1.0 : 0.0;
+        if (tid == 0) {
+            double dm = 0;
+            for (int i = 0; i < k; i++) dm = fmax(dm, diag0[i]);
+            cinfo[kMaxKp] = dm;
+        }
+    }
     if (bad && mode == 0) {
         __syncthreads();
         for (int e = tid; e < kp * kp; e += 256) {
@@ -175,7 +194,7 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
             flags[0] = 1;
         }
         __syncthreads();
-        chol_factor(A, diag0, k, &bad);
+        chol_factor(A, diag0, k, &bad, colbad);
     }
     // in-place inverse of the upper triangle, one column at a time
     for (int j = 0; j < k; j++) {
@@ -199,10 +218,53 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
 }
 
 hipError_t launch_chol_inv(hipStream_t st, const double *G, int k, int kp, double shift_mult,
-                           int mode, double *Rinv, int *flags)
+                           int mode, double *Rinv, int *flags, double *cinfo)
 {
     if (kp > kMaxKp || k > kp) return hipErrorInvalidValue;
-    k_chol_inv<<<1, 256, 0, st>>>(G, k, kp, shift_mult, mode, Rinv, flags);
+    k_chol_inv<<<1, 256, 0, st>>>(G, k, kp, shift_mult, mode, Rinv, flags, cinfo);
+    return hipGetLastError();
+}
+
+// ------------------------------------------- dependent-column replacement --
+// A (Mpad x kp, row-major): every column j < k that the last mode-0 factor
+// dropped (cinfo[j] != 0) is overwritten by a fixed pseudo-random vector
+// (splitmix64 of the row and column, uniform in [-1, 1), rows < M only; the
+// padding rows stay zero) scaled to the largest column norm of A.  A basis
+// whose columns span less than k dimensions (k at or above the rank of
+// A A^T, e.g. k = M with centred columns) is thereby completed with
+// directions in general position, as Householder QR completes it with
+// directions chosen by its rounding errors.
+__device__ inline uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+__global__ __launch_bounds__(256) void k_replace_cols(double *__restrict__ A, int M, int64_t Mpad,
+                                                      int k, int kp,
+                                                      const double *__restrict__ cinfo)
+{
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= Mpad * kp) return;
+    const int64_t i = idx / kp;
+    const int j = (int)(idx % kp);
+    if (j >= k || cinfo[j] == 0.0) return;
+    double v = 0.0;
+    if (i < M) {
+        const uint64_t h = splitmix64(((uint64_t)i << 7) ^ (uint64_t)j);
+        v = ((double)(h >> 11) * 0x1.0p-53 * 2.0 - 1.0) * sqrt(3.0 * cinfo[kMaxKp] / M);
+    }
+    A[idx] = v;
+}
+
+hipError_t launch_replace_cols(hipStream_t st, double *A, int M, int64_t Mpad, int k, int kp,
+                               const double *cinfo)
+{
+    if (kp > kMaxKp || k > kp) return hipErrorInvalidValue;
+    const unsigned nb = (unsigned)((Mpad * kp + 255) / 256);
+    k_replace_cols<<<nb, 256, 0, st>>>(A, M, Mpad, k, kp, cinfo);
     return hipGetLastError();
 }
 

@@ -78,6 +78,7 @@ typedef struct dpca_report {
     double t_h2d;           /* streamed: seconds of host-to-device copies    */
     double h2d_bytes;       /* streamed: bytes copied host-to-device         */
     int host_mode;          /* streamed: the DPCA_HOST_* mode in use         */
+    int qr_repaired;        /* rank-deficient bases completed (k >= rank)     */
 } dpca_report;
 
 /* Defaults of initialize_structure (src/utilities.cpp:195-232). */

@@ -58,3 +58,45 @@ def test_cli_about_runs_without_gpu():
     exe = D.EXE_PATH
     out = subprocess.run([exe, "-about"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 0 and "DistPCA" in out.stdout
+
+
+def _cli(args):
+    return subprocess.run([D.EXE_PATH] + args, capture_output=True, text=True, timeout=60)
+
+
+@pytest.fixture(scope="module")
+def tiny_bfile(tmp_path_factory):
+    pre = str(tmp_path_factory.mktemp("cli") / "tiny")
+    D.write_synthetic(pre, D.gen_spec(40, 300, npop=2, fst=0.05, missing=0.0, seed=3))
+    return pre
+
+
+def test_cli_missing_files(tmp_path, tiny_bfile):
+    # src/distpca.cpp:105-125: the .bed is opened first, then the .fam, .bim
+    out = _cli(["-bfile", str(tmp_path / "none")])
+    assert out.returncode == 1
+    assert f"[Data::read_bed] Error reading file {tmp_path / 'none'}.bed" in out.stdout
+    pre = str(tmp_path / "nofam")
+    os.symlink(tiny_bfile + ".bed", pre + ".bed")
+    out = _cli(["-bfile", pre])
+    assert out.returncode == 1
+    assert f"Error reading file '{pre}.fam': No such file or directory" in out.stdout
+    os.symlink(tiny_bfile + ".fam", pre + ".fam")
+    out = _cli(["-bfile", pre])
+    assert out.returncode == 1
+    assert f"Error reading file '{pre}.bim': No such file or directory" in out.stdout
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["-nsv", "0"], "NSV was either zero or negative. Aborting..."),
+    (["-nsv", "-3"], "NSV was either zero or negative. Aborting..."),
+    (["-crit", "3"], "crit must be 0, 1 or 2"),
+    (["-precision", "half"], "Unknown -precision 'half' (fp64 | mixed)"),
+    (["-hostmem", "disk"], "Unknown -hostmem 'disk' (pinned | pageable | file)"),
+])
+def test_cli_parameter_errors_before_any_device_call(tiny_bfile, args, msg):
+    # parameters are resolved before the device is touched, so these fail
+    # the same way with or without a GPU
+    out = _cli(["-bfile", tiny_bfile] + args)
+    assert out.returncode == 1
+    assert msg in out.stdout, out.stdout

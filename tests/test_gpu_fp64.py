@@ -118,6 +118,15 @@ def test_orthonormalize_cholqr2():
     # Householder QR (numpy) is the yardstick for that loss
     qn, _ = np.linalg.qr(B)
     assert O.principal_angles(U, Q2).max() <= 10 * O.principal_angles(U, qn).max() + 1e-12
+    # exactly rank-deficient (rank 25 of 32, dependent columns interleaved):
+    # dgeqrf + dorgqr never fail; the library completes the span with
+    # orthonormal directions as well
+    C = rng.standard_normal((3000, 25))
+    mix = rng.standard_normal((25, 7))
+    D_ = np.concatenate([C[:, :10], C @ mix, C[:, 10:]], axis=1)
+    Q3, _ = D.orthonormalize(D_)
+    assert np.abs(Q3.T @ Q3 - np.eye(32)).max() < 1e-12
+    assert np.abs(Q3 @ (Q3.T @ D_) - D_).max() < 1e-10 * np.abs(D_).max()
 
 
 @pytest.mark.parametrize("k", [1, 7, 20, 40, 64])
@@ -244,6 +253,56 @@ def test_cli_dropin(tmp_path):
         a = O.principal_angles(U[:, :j], r.rhs[:, :j]).max()
         sp = O.principal_angles(ra.rhs[:, :j], r.rhs[:, :j]).max()
         assert a <= max(1e-8, 20 * sp), (j, a, sp)
+
+
+@pytest.mark.parametrize("incore", [True, False])
+def test_rank_deficient_basis(incore):
+    # nrhs = M: centred columns leave A A^T rank M - 1, so every basis the
+    # iteration orthonormalises spans at most M - 1 dimensions; the reference's
+    # Householder QR completes it, the library replaces the dependent column
+    # and reports the completion
+    _banner(f"rank-deficient basis (nrhs = M), incore {incore}")
+    M, N = 30, 3000
+    packed = _packed_random(M, N, seed=21)
+    p = O.Params(nsv=10, nrhs=30, power=2, crit=1, tol=1e-12, miter=6,
+                 bsize=N if incore else 1000)
+    ref = O.rsi(packed, M, N, O.resolve(p, M, N))
+    with D.PCA.from_packed(packed, M, N, D.Params(**vars(p))) as pca:
+        sigma, rhs, rep = pca.run()
+        assert rep["qr_repaired"] > 0
+        s2, _, rep2 = pca.run()          # robust from the start on this context
+        assert rep2["qr_repaired"] > 0 and np.array_equal(s2, sigma)
+    assert rep["total_its"] == ref.total_its
+    rel = np.abs(sigma[:9] ** 2 - ref.sigma[:9] ** 2) / ref.sigma[:9] ** 2
+    assert rel.max() < 1e-10, rel
+    assert O.principal_angles(rhs[:, :9], ref.rhs[:, :9]).max() < 1e-8
+
+
+@pytest.mark.parametrize("flags,msgs,nsv,nrhs", [
+    # src/distpca.cpp:272-352 on a 40 x 2000 cohort (min(M, N) = 40)
+    (["-nsv", "50", "-nrhs", "0", "-bsize", "5000"],
+     ["Warning: Specified block size (5000) exceeds local SNPs (2000). Adjusting to local_N.",
+      "The value of NSV given was larger than min(M, N). Adjusting to NSV = min(M,N)...",
+      "NRHS was either zero or negative. Adjusting to NRHS = min(2xNSV, min(M,N))..."], 40, 40),
+    (["-nsv", "6", "-nrhs", "3"],
+     ["NRHS can not be smaller than NSV. Adjusting to NRHS = min(2xNSV,min(M,N))..."], 6, 12),
+    (["-nsv", "4", "-nrhs", "60"],
+     ["The value of NRHS given was larger than min(M,N). Adjusting to NRHS = min(M,N)..."],
+     4, 40),
+])
+def test_cli_clamp_messages(tmp_path, flags, msgs, nsv, nrhs):
+    _banner(f"CLI clamps {flags}")
+    pre = str(tmp_path / "clamp")
+    D.write_synthetic(pre, D.gen_spec(40, 2000, npop=2, fst=0.05, missing=0.01, seed=2))
+    out = subprocess.run([D.EXE_PATH, "-bfile", pre, "-miter", "3"] + flags,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    for m in msgs:
+        assert m in out.stdout, (m, out.stdout)
+    for m in ("Estimated memory per rank:", "Block size (in MB):", "Memory per rank:"):
+        assert m in out.stdout
+    assert f"Total number of singular pairs sought: {nsv}\n" in out.stdout
+    assert f"Total number of right-hand sides used: {nrhs}\n" in out.stdout
 
 
 @pytest.mark.parametrize("precision", [D.FP64, D.MIXED])
