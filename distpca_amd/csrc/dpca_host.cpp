@@ -382,7 +382,7 @@ static int alloc_work(dpca_ctx *c)
         // length), 4 or 2 SNP tiles per wave depending on how many
         // workgroups that leaves.
         const int64_t nsegs = (c->ldp + 4095) / 4096;
-        c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 64);
+        c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 128);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
         c->st_tiles = za_i8_tiles(c->k);
         if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 1, 2

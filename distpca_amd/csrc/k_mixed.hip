@@ -19,12 +19,6 @@ namespace dpca {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// Spread the four 2-bit codes of byte b into the four bytes of a dword.
-__device__ __forceinline__ uint32_t spread4(uint32_t b)
-{
-    uint32_t v = (b | (b << 12)) & 0x000F000Fu;
-    return (v | (v << 6)) & 0x03030303u;
-}
 
 // ----------------------------------------------------- T segments -> T -----
 // T[s][c] = sum of the sample-segment partials of Z Q, in FP64.
@@ -301,15 +295,18 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
     // current one is multiplied, then stored into the other LDS buffer
     uint32_t rpk[NPK][4];
     u32x4 ruw[NUW];
-#define DPCA_ZT8_LOAD(c0)                                                                   \
+#define DPCA_ZT8_LOADP(c0, dst)                                                             \
     do {                                                                                    \
         _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
             const int it_ = tid + 512 * u;                                                  \
             const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
             const uint8_t *p_ = packed + ((c0) + 4 * sg_) * ldp + mb + 4 * bq_;             \
-            _Pragma("unroll") for (int j = 0; j < 4; j++) rpk[u][j] =                      \
+            _Pragma("unroll") for (int j = 0; j < 4; j++) dst[u][j] =                      \
                 *reinterpret_cast<const uint32_t *>(p_ + j * ldp);                          \
         }                                                                                   \
+    } while (0)
+#define DPCA_ZT8_LOADU(c0)                                                                  \
+    do {                                                                                    \
         const u32x4 *q_ = UW8 + ((c0) / 8) * VT * 16;                                       \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
             const int it_ = tid + 512 * u;                                                  \
@@ -338,8 +335,12 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             if (it_ < NUWT) uw[buf][it_] = ruw[u];                                          \
         }                                                                                   \
     } while (0)
-    DPCA_ZT8_LOAD(sbeg < send ? sbeg : 0);
-    DPCA_ZT8_STORE(0);
+    {
+        const int64_t c0 = sbeg < send ? sbeg : 0;
+        DPCA_ZT8_LOADP(c0, rpk);
+        DPCA_ZT8_LOADU(c0);
+        DPCA_ZT8_STORE(0);
+    }
     __syncthreads();
 
     int cur = 0;
@@ -347,7 +348,8 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
         {
             // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
-            DPCA_ZT8_LOAD(nx);
+            DPCA_ZT8_LOADP(nx, rpk);
+            DPCA_ZT8_LOADU(nx);
         }
 #pragma unroll
         for (int t = 0; t < SC / 32; t++) {
@@ -377,7 +379,8 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
         __syncthreads();
     }
 #undef DPCA_ZT8_STORE
-#undef DPCA_ZT8_LOAD
+#undef DPCA_ZT8_LOADU
+#undef DPCA_ZT8_LOADP
     float *out = part + ((int64_t)blockIdx.y * Mpad + 4 * mb + wave * SA * 16) * KP;
 #pragma unroll
     for (int oc = 0; oc < CT; oc++) {
@@ -401,9 +404,10 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
 // written as an FP32 partial of T.
 //
 // Q8 layout: per 16-sample group G, per virtual tile v, per lane r one
-// 16-byte word (byte e = sample 16G + e) of the slice and column of (v, r),
-// at Q8[(G * VT + v) * 16 + r]; qmax[seg * kp + c] the column's max |Q| over
-// the segment.
+// 16-byte word of the slice and column of (v, r), at Q8[(G * VT + v) * 16 + r];
+// byte e holds sample 16G + 4 (e & 3) + (e >> 2), the order in which k_za_i8
+// unpacks a packed dword (its dword q holds the bit pairs q of the four
+// bytes); qmax[seg * kp + c] the column's max |Q| over the segment.
 __global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int F, int R, int VT,
                      int64_t seg_samples, const unsigned *__restrict__ qmax,
                      u32x4 *__restrict__ Q8)
@@ -421,7 +425,7 @@ __global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int F, 
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             int q[3];
-            slices3(Q[(G * 16 + e) * kp + c], inv, q);
+            slices3(Q[(G * 16 + 4 * (e & 3) + (e >> 2)) * kp + c], inv, q);
             w[e >> 2] |= (uint32_t)(q[j] & 0xFF) << (8 * (e & 3));
         }
     }
@@ -446,9 +450,9 @@ hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int k
 }
 
 // Workgroup: 8 waves x ST tiles x 16 SNPs, one sample segment (blockIdx.y).
-// Per 256-sample chunk (64 packed bytes per row), lane group g owns the 64
-// samples of its 16 packed bytes; at K step t it supplies samples
-// 64g + 16t .. +15.  The chunk's Q8 words are staged in LDS.
+// Per chunk of CB packed bytes per row (4 CB samples), lane group g owns the
+// CB samples of its CB / 4 packed bytes; at K step t it supplies samples
+// CB g + 16t .. +15.  The chunk's Q8 words are staged in LDS.
 template <int ST, int F, int RT>
 __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ Q8,
@@ -460,7 +464,10 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     constexpr int VT = 3 * F + RT;
     constexpr int CT = F + (RT > 0 ? 1 : 0);
     constexpr int KP = CT * 16;
-    constexpr int CB = 64;                      // packed bytes per chunk
+    // packed bytes per chunk: 128 where two LDS buffers of its Q8 words
+    // (CB * VT * 64 bytes each) fit the 160 KB of LDS, else 64
+    constexpr int CB = (VT <= 10) ? 128 : 64;
+    constexpr int NW = CB / 64;                 // 16-byte words per lane and row
     constexpr int NQT = (CB / 4) * VT * 16;     // Q8 words per chunk
     constexpr int NQ = (NQT + 511) / 512;
     __shared__ u32x4 qs[2][NQT];               // double buffer, one barrier per chunk
@@ -484,7 +491,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     // software pipeline: the next chunk's Q8 words and packed bytes are
     // loaded into registers while the current chunk is multiplied; the Q8
     // words then go to the other LDS buffer
-    u32x4 rq[NQ], rp[ST];
+    u32x4 rq[NQ], rp[ST][NW];
 #define DPCA_ZA8_LOAD(c0)                                                                   \
     do {                                                                                    \
         const u32x4 *src_ = Q8 + ((c0) / 4) * VT * 16;                                      \
@@ -492,9 +499,10 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
             const int it_ = tid + 512 * u;                                                  \
             if (it_ < NQT) rq[u] = src_[it_];                                               \
         }                                                                                   \
-        _Pragma("unroll") for (int s = 0; s < ST; s++) rp[s] =                             \
-            *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + (c0) +     \
-                                             16 * g);                                       \
+        _Pragma("unroll") for (int s = 0; s < ST; s++)                                     \
+            _Pragma("unroll") for (int n = 0; n < NW; n++) rp[s][n] =                      \
+                *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + (c0) + \
+                                                 (CB / 4) * g + 16 * n);                    \
     } while (0)
 #define DPCA_ZA8_STORE(buf)                                                                 \
     do {                                                                                    \
@@ -505,33 +513,70 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     } while (0)
     DPCA_ZA8_LOAD(b0 < b1 ? b0 : 0);
     DPCA_ZA8_STORE(0);
+    // where the registers allow, the packed codes (HBM) run two chunks
+    // ahead and the Q8 words (L2) one
+    constexpr bool PF2 = ST * VT <= 16;
+    u32x4 rp2[ST][NW];
+    if constexpr (PF2) {
+        const int64_t c2 = (b0 + CB < b1) ? b0 + CB : b0;
+#pragma unroll
+        for (int s = 0; s < ST; s++)
+#pragma unroll
+            for (int n = 0; n < NW; n++)
+                rp2[s][n] = *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + c2 +
+                                                             (CB / 4) * g + 16 * n);
+    }
     __syncthreads();
 
     int cur = 0;
     for (int64_t cb = b0; cb < b1; cb += CB, cur ^= 1) {
-        u32x4 pk[ST];
+        u32x4 pk[ST][NW];
 #pragma unroll
-        for (int s = 0; s < ST; s++) pk[s] = rp[s];
+        for (int s = 0; s < ST; s++)
+#pragma unroll
+            for (int n = 0; n < NW; n++) pk[s][n] = rp[s][n];
         {
+            // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (cb + CB < b1) ? cb + CB : cb;
-            DPCA_ZA8_LOAD(nx);
+            if constexpr (PF2) {
+                const int64_t n2 = (cb + 2 * CB < b1) ? cb + 2 * CB : nx;
+                const u32x4 *src_ = Q8 + (nx / 4) * VT * 16;
+#pragma unroll
+                for (int u = 0; u < NQ; u++) {
+                    const int it_ = tid + 512 * u;
+                    if (it_ < NQT) rq[u] = src_[it_];
+                }
+#pragma unroll
+                for (int s = 0; s < ST; s++)
+#pragma unroll
+                    for (int n = 0; n < NW; n++) {
+                        rp[s][n] = rp2[s][n];
+                        rp2[s][n] = *reinterpret_cast<const u32x4 *>(
+                            packed + (base + s * 16 + r) * ldp + n2 + (CB / 4) * g + 16 * n);
+                    }
+            } else {
+                DPCA_ZA8_LOAD(nx);
+            }
         }
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
+        for (int t = 0; t < 4 * NW; t++) {
+            // dword q of the A operand: bit pair q of each of the four
+            // packed bytes, i.e. samples 4b + q for b = 0..3 (Q8 bytes follow
+            // the same order), one shift and mask per four codes
             u32x4 ah[ST], am[ST];
 #pragma unroll
             for (int s = 0; s < ST; s++) {
-                const uint32_t w = pk[s][t];
+                const uint32_t w = pk[s][t / 4][t % 4];
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const uint32_t cw = spread4((w >> (8 * k)) & 0xFFu);
-                    ah[s][k] = __builtin_amdgcn_perm(0u, HT, cw);
-                    am[s][k] = __builtin_amdgcn_perm(0u, MT, cw);
+                for (int q = 0; q < 4; q++) {
+                    const uint32_t cw = (w >> (2 * q)) & 0x03030303u;
+                    ah[s][q] = __builtin_amdgcn_perm(0u, HT, cw);
+                    am[s][q] = __builtin_amdgcn_perm(0u, MT, cw);
                 }
             }
 #pragma unroll
             for (int v = 0; v < VT; v++) {
-                const u32x4 b = qs[cur][((4 * g + t) * VT + v) * 16 + r];
+                const u32x4 b = qs[cur][(((CB / 16) * g + t) * VT + v) * 16 + r];
 #pragma unroll
                 for (int s = 0; s < ST; s++) {
                     acch[s][v] = mfma_i8(ah[s], b, acch[s][v]);
@@ -631,7 +676,9 @@ hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int
 {
     const ColTiles ct = col_tiles(k);
     const int64_t per = 8 * 16 * (int64_t)st_tiles;
-    if (rows % per || seg_bytes % 64 || ldp % 64 || seg_bytes > 16384) return hipErrorInvalidValue;
+    // |h-sum| <= 2 * 127 per sample: a segment of <= 2^23 samples stays exact in int32
+    if (rows % per || seg_bytes % 128 || ldp % 128 || seg_bytes > (1 << 21))
+        return hipErrorInvalidValue;
     dim3 grid((unsigned)(rows / per), (unsigned)nseg);
     const u32x4 *q = reinterpret_cast<const u32x4 *>(Q8);
 #define X(F_, RT_)                                                                          \
