@@ -156,6 +156,20 @@ __device__ __forceinline__ double combine(const i32x4 (&acc)[3 * F + RT], int oc
     return (double)s0 + ldexp((double)s1, -7) + ldexp((double)s2, -14);
 }
 
+// Same from slices 0 and 1 only (the slice-2 tiles of a product that needs
+// 2^-7 less precision are not computed).
+template <int F, int RT>
+__device__ __forceinline__ double combine2(const i32x4 (&acc)[3 * F + RT], int oc, int q, int r,
+                                           int R, int g)
+{
+    if (oc < F) return (double)acc[3 * oc][q] + ldexp((double)acc[3 * oc + 1][q], -7);
+    const int x = r < R ? r : 0;
+    const int s0 = rem_slice<F, RT>(acc, q, 0, x, R, g);
+    const int s1 = rem_slice<F, RT>(acc, q, 1, x, R, g);
+    if (r >= R) return 0.0;
+    return (double)s0 + ldexp((double)s1, -7);
+}
+
 // smax[split * kp + c] = max over rows of the split of f_s |T_sc|, with
 // f_s = |d_s| max(1, |mu_s|) (covers |U| = |d T| and |W| = |d mu T|), or
 // f_s = 1 when dsc is null.
@@ -453,7 +467,7 @@ hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int k
 // Per chunk of CB packed bytes per row (4 CB samples), lane group g owns the
 // CB samples of its CB / 4 packed bytes; at K step t it supplies samples
 // CB g + 16t .. +15.  The chunk's Q8 words are staged in LDS.
-template <int ST, int F, int RT>
+template <int ST, int F, int RT, bool SKR>
 __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ Q8,
                                                const unsigned *__restrict__ qmax,
@@ -477,7 +491,23 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     const int64_t b0 = (int64_t)blockIdx.y * seg_bytes;
     int64_t b1 = b0 + seg_bytes;
     if (b1 > ldp) b1 = ldp;
-    const uint32_t HT = 0x02010000u, MT = 0x01010001u;  // h, m by 2-bit code
+    const uint32_t MT = 0x01010001u;  // m by 2-bit code
+    // Per-row A values 63 h - mh with mh = rint(63 mu) (exact int8, 0 when
+    // missing), as a byte table by 2-bit code: T_s = d_s (sum (63 h - mh m) Q
+    // / 63 - delta_s sum m Q), delta_s = mu_s - mh / 63, |delta_s| <= 1/126,
+    // so the mask product needs only slices 0 and 1 (2^-7 less precision).
+    uint32_t tab[ST];
+#pragma unroll
+    for (int s = 0; s < ST; s++) {
+        const int mh = (int)rint(63.0 * mu[base + s * 16 + r]);  // mu in [0, 2]
+        tab[s] = ((uint32_t)(-mh) & 0xFFu) | (((uint32_t)(63 - mh) & 0xFFu) << 16) |
+                 (((uint32_t)(126 - mh) & 0xFFu) << 24);
+    }
+    // tiles holding only slice 2 (of whole columns, and the last remainder
+    // tile when SKR) are left out of the mask product
+    auto slice2_only = [](int v) constexpr {
+        return v < 3 * F ? v % 3 == 2 : (SKR && v == 3 * F + RT - 1);
+    };
 
     i32x4 acch[ST][VT], accm[ST][VT];
 #pragma unroll
@@ -570,7 +600,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const uint32_t cw = (w >> (2 * q)) & 0x03030303u;
-                    ah[s][q] = __builtin_amdgcn_perm(0u, HT, cw);
+                    ah[s][q] = __builtin_amdgcn_perm(0u, tab[s], cw);
                     am[s][q] = __builtin_amdgcn_perm(0u, MT, cw);
                 }
             }
@@ -580,7 +610,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 #pragma unroll
                 for (int s = 0; s < ST; s++) {
                     acch[s][v] = mfma_i8(ah[s], b, acch[s][v]);
-                    accm[s][v] = mfma_i8(am[s], b, accm[s][v]);
+                    if (!slice2_only(v)) accm[s][v] = mfma_i8(am[s], b, accm[s][v]);
                 }
             }
         }
@@ -599,9 +629,11 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int64_t row = base + s * 16 + 4 * g + q;
-                const double h = combine<F, RT>(acch[s], oc, q, r, R, g);
-                const double m = combine<F, RT>(accm[s], oc, q, r, R, g);
-                out[row * KP + col] = (float)(dsc[row] * sc * (h - mu[row] * m));
+                const double a = combine<F, RT>(acch[s], oc, q, r, R, g);
+                const double m = combine2<F, RT>(accm[s], oc, q, r, R, g);
+                const double mr = mu[row];
+                const double delta = mr - rint(63.0 * mr) / 63.0;
+                out[row * KP + col] = (float)(dsc[row] * sc * (a / 63.0 - delta * m));
             }
     }
 }
@@ -625,7 +657,7 @@ static hipError_t zt_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_
     }
 }
 
-template <int ST, int F, int RT>
+template <int ST, int F, int RT, bool SKR>
 static hipError_t za_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
                         const u32x4 *Q8, const unsigned *qmax, const double *mu,
                         const double *dsc, int64_t seg_bytes, int64_t rows, int R, float *Tpart)
@@ -633,7 +665,7 @@ static hipError_t za_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_
     if constexpr (ST * (3 * F + RT) > 18) {
         return hipErrorInvalidValue;  // would not fit the registers
     } else {
-        k_za_i8<ST, F, RT><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows,
+        k_za_i8<ST, F, RT, SKR><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows,
                                                  R, Tpart);
         return hipGetLastError();
     }
@@ -676,19 +708,28 @@ hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int
 {
     const ColTiles ct = col_tiles(k);
     const int64_t per = 8 * 16 * (int64_t)st_tiles;
-    // |h-sum| <= 2 * 127 per sample: a segment of <= 2^23 samples stays exact in int32
-    if (rows % per || seg_bytes % 128 || ldp % 128 || seg_bytes > (1 << 21))
+    // |A| <= 126 per sample: a segment of <= 2^16 samples stays exact in int32
+    if (rows % per || seg_bytes % 128 || ldp % 128 || seg_bytes > 16384)
         return hipErrorInvalidValue;
     dim3 grid((unsigned)(rows / per), (unsigned)nseg);
     const u32x4 *q = reinterpret_cast<const u32x4 *>(Q8);
+    // the last remainder tile holds only slice 2 when the slices 0 and 1 of
+    // the R remainder columns fit the tiles before it
+    const bool skr = ct.RT > 0 && 2 * ct.R <= 16 * (ct.RT - 1);
 #define X(F_, RT_)                                                                          \
     if (ct.F == F_ && ct.RT == RT_) {                                                        \
+        if (st_tiles == 2 && skr)                                                            \
+            return za_go<2, F_, RT_, true>(st, grid, packed, ldp, q, qmax, mu, dsc,          \
+                                           seg_bytes, rows, ct.R, Tpart);                    \
         if (st_tiles == 2)                                                                   \
-            return za_go<2, F_, RT_>(st, grid, packed, ldp, q, qmax, mu, dsc, seg_bytes,     \
-                                     rows, ct.R, Tpart);                                     \
+            return za_go<2, F_, RT_, false>(st, grid, packed, ldp, q, qmax, mu, dsc,         \
+                                            seg_bytes, rows, ct.R, Tpart);                   \
+        if (st_tiles == 1 && skr)                                                            \
+            return za_go<1, F_, RT_, true>(st, grid, packed, ldp, q, qmax, mu, dsc,          \
+                                           seg_bytes, rows, ct.R, Tpart);                    \
         if (st_tiles == 1)                                                                   \
-            return za_go<1, F_, RT_>(st, grid, packed, ldp, q, qmax, mu, dsc, seg_bytes,     \
-                                     rows, ct.R, Tpart);                                     \
+            return za_go<1, F_, RT_, false>(st, grid, packed, ldp, q, qmax, mu, dsc,         \
+                                            seg_bytes, rows, ct.R, Tpart);                   \
         return hipErrorInvalidValue;                                                         \
     }
     DPCA_I8_SHAPES(X)
