@@ -26,6 +26,8 @@ WORKLOADS = {
     "c1": (1024, 50000, 3, 0.05, 0.0, 0.01, 0, 10, 20, 4, 4096),
     "c2": (20000, 500000, 5, 0.01, 0.0, 0.01, 1, 10, 20, 4, 16384),
     "c3": (100000, 1000000, 8, 0.005, 0.2, 0.02, 2, 20, 40, 6, 0),
+    # one GPU's SNP shard of the 100k x 1M cohort split over 8 GPUs
+    "c3-shard8": (100000, 125000, 8, 0.005, 0.2, 0.02, 2, 20, 40, 6, 0),
     # out-of-core: host memory (pinned, or pageable staged through pinned
     # blocks when the driver cannot pin it all), 16 GB device cap, ~512 MB
     # blocks; the second is the same pipeline at 1/8 of the SNPs (2 GB cap)
