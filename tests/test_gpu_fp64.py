@@ -189,6 +189,32 @@ def test_run_c1_synthetic(c1_golden):
     assert O.principal_angles(rhs[:, :2], c1_golden["rhs"][:, :2]).max() < 1e-8
 
 
+@pytest.mark.parametrize("incore", [False, True])
+def test_crit0_trace_criterion(tmp_path, incore):
+    # crit 0: |delta trace| / trace <= tol (src/methods.cpp:1025-1031); the
+    # oracle's trajectory crosses tol 3e-4 between 3.30e-4 and 2.66e-4, so
+    # both runs stop after 12 iterations
+    _banner(f"crit 0 run, incore {incore}")
+    pre = str(tmp_path / "crit0")
+    D.write_synthetic(pre, D.gen_spec(600, 8000, npop=3, fst=0.05, missing=0.01, seed=21))
+    M, N, _, _, packed = O.read_plink(pre)
+    kw = dict(nsv=5, nrhs=10, power=2, crit=0, tol=3e-4, miter=40, bsize=N if incore else 1000)
+    p = O.resolve(O.Params(**kw), M, N)
+    ref = O.rsi_incore(O.dense_z(packed, N, M), p) if incore else O.rsi(packed, M, N, p)
+    with D.PCA.from_bfile(pre, D.Params(**kw)) as pca:
+        sigma, rhs, rep = pca.run()
+        crit = pca.crit_history()
+    assert rep["total_its"] == ref.total_its == 12 and rep["incore"] == int(incore)
+    # |delta trace| / trace differences two traces that agree to ~1e-3, so
+    # the summation-order differences of the noise components (~1e-12) show
+    # up ~1e3 larger; the stop itself has a 10% margin
+    dev = np.abs(np.asarray(crit) - ref.crit_values) / np.asarray(ref.crit_values)
+    print("crit 0 trajectory max rel dev", dev.max(), flush=True)
+    assert dev.max() < 1e-6
+    assert np.allclose(sigma, ref.sigma, rtol=1e-10)
+    assert O.principal_angles(rhs[:, :2], ref.rhs[:, :2]).max() < 1e-8
+
+
 @pytest.mark.parametrize("precision", [D.FP64, D.MIXED])
 def test_pass_saving_schedule(c1_golden, precision):
     # A A^T (Q X) = (A A^T Q) X: from the second iteration on the first pass
