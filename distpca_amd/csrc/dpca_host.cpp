@@ -378,19 +378,20 @@ static int alloc_work(dpca_ctx *c)
     if (c->p.precision == DPCA_FP64) {
         r |= dalloc(&c->part, (size_t)c->splits * mk);
     } else {
-        // T = Z Q: sample segments of <= 16384 samples (FP32 accumulation
-        // length), 4 or 2 SNP tiles per wave depending on how many
-        // workgroups that leaves.
-        const int64_t nsegs = (c->ldp + 4095) / 4096;
+        // T = Z Q: sample segments of <= 65536 samples (the int32 bound of
+        // the centred rows; fewer segments, fewer FP32 partials to reduce),
+        // 2 or 1 SNP tiles per wave by column tiles.
+        const int64_t nsegs = (c->ldp + 16383) / 16384;
         c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 128);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
         c->st_tiles = za_i8_tiles(c->k);
         if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 1, 2
-        // Z^T T: SNP splits of <= 16384 rows, >= ~1024 workgroups
+        // Z^T T: SNP splits of <= 32768 rows (fewer partials), >= ~1024
+        // workgroups
         c->sa_tiles = zt_i8_tiles(c->k, c->ldp);
         if (const char *e = getenv("DPCA_ZT_TILES")) c->sa_tiles = atoi(e);  // tuning knob: 2, 4
         const int64_t tl = c->ldp / (c->sa_tiles * 32);
-        int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 16383) / 16384);
+        int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 32767) / 32768);
         ns = std::min<int64_t>(ns, B / 128);
         c->rps = round_up((B + ns - 1) / ns, 128);
         c->nsplit = (int)((B + c->rps - 1) / c->rps);
