@@ -47,6 +47,7 @@ class Report(ctypes.Structure):
         ("wall", ctypes.c_double), ("last_crit", ctypes.c_double),
         ("t_h2d", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
         ("host_mode", ctypes.c_int), ("qr_repaired", ctypes.c_int),
+        ("resident_bytes", ctypes.c_double),
     ]
 
     def as_dict(self):
@@ -303,9 +304,9 @@ class PCA:
     def bench_pass(self, Q: np.ndarray, npass: int):
         """Event-timed passes: (ms per pass, ms Z Q kernel, ms Z^T T kernel, ms other,
         host wall ms per pass, host-to-device copy ms per pass, small-kernel ms,
-        host mode of a streamed matrix or -1)."""
+        host mode of a streamed matrix or -1, host-to-device bytes per pass)."""
         Q = np.ascontiguousarray(Q, dtype=np.float64)
-        ms = np.zeros(8, np.float64)
+        ms = np.zeros(9, np.float64)
         _check(lib().dpca_bench_pass(self._h, _ptr(Q), npass, _ptr(ms)))
         return ms
 

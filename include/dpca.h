@@ -79,6 +79,8 @@ typedef struct dpca_report {
     double h2d_bytes;       /* streamed: bytes copied host-to-device         */
     int host_mode;          /* streamed: the DPCA_HOST_* mode in use         */
     int qr_repaired;        /* rank-deficient bases completed (k >= rank)     */
+    double resident_bytes;  /* streamed: leading blocks kept in HBM after the
+                               first pass (within device_cap)                 */
 } dpca_report;
 
 /* Defaults of initialize_structure (src/utilities.cpp:195-232). */
@@ -174,7 +176,8 @@ int dpca_syev(const double *B, int k, double *w, double *X, int device);
  * stream drain), ms[5] host-to-device copy time per pass (streamed
  * matrices; it overlaps the kernels), ms[6] kernel time of the small pass
  * kernels, ms[7] the DPCA_HOST_* mode of a streamed matrix (-1 when
- * resident).  ms must hold 8 values. */
+ * resident), ms[8] bytes copied host-to-device per pass (not a time).
+ * ms must hold 9 values. */
 int dpca_bench_pass(dpca_ctx *c, const double *Q, int npass, double *ms);
 
 void dpca_close(dpca_ctx *c);
