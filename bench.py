@@ -36,8 +36,11 @@ WORKLOADS = {
     # blocks; the second is the same pipeline at 1/8 of the SNPs (2 GB cap)
     "c4": (400000, 2000000, 10, 0.005, 0.0, 0.01, 3, 20, 40, 4, 65536),
     "c4s": (400000, 250000, 10, 0.005, 0.0, 0.01, 3, 20, 40, 4, 65536),
+    # the 20k x 500k cohort written to a .bed file and re-read from it every
+    # pass (the reference's on-disk block loop), 0.5 GB device cap
+    "c2-file": (20000, 500000, 5, 0.01, 0.0, 0.01, 1, 10, 20, 4, 16384),
 }
-STREAMED = {"c4": 16e9, "c4s": 16e9}
+STREAMED = {"c4": 16e9, "c4s": 16e9, "c2-file": 0.5e9}
 PEAK_INT8 = 5.0e15   # dense int8 MFMA, ops/s
 PEAK_FP64 = 78.6e12
 PEAK_HBM = 8.0e12
@@ -262,14 +265,30 @@ def main():
     np_ = (M + 3) // 4
     packed_bytes = N * np_
 
-    log(f"generating {a.workload}: {M} x {N} ({packed_bytes / 1e9:.2f} GB packed) in HBM")
     t0 = time.perf_counter()
     cap = int(STREAMED.get(a.workload, 0))
     if cap and a.workload == "c4s":
         cap = int(2e9)   # same block pipeline on a matrix 8x smaller than c4
+    bed = None
+    if a.workload.endswith("-file"):
+        import tempfile
+        tmp = tempfile.TemporaryDirectory()
+        bed = os.path.join(tmp.name, a.workload)
+        log(f"writing {a.workload}: {M} x {N} ({packed_bytes / 1e9:.2f} GB) to {bed}.bed")
+        D.write_synthetic(bed, spec)
+        log(f"written in {time.perf_counter() - t0:.1f} s")
+    else:
+        log(f"generating {a.workload}: {M} x {N} ({packed_bytes / 1e9:.2f} GB packed) in HBM")
+
+    def open_ctx(params):
+        if bed:
+            params.host_mode = D.HOST_FILE
+            return D.PCA.from_bfile(bed, params)
+        return D.PCA.synthetic(spec, params)
+
     pm = D.Params(nsv=nsv, nrhs=nrhs, power=power, bsize=bsize, precision=D.MIXED,
                   device_cap=cap)
-    mixed = D.PCA.synthetic(spec, pm)
+    mixed = open_ctx(pm)
     log(f"generated + stats in {time.perf_counter() - t0:.1f} s")
     Q = np.linalg.qr(D.omega(M, nrhs))[0]
 
@@ -331,7 +350,7 @@ def main():
         log(f"fp64: {a.fp64_steps} passes")
         pf = D.Params(nsv=nsv, nrhs=nrhs, power=power, bsize=bsize, precision=D.FP64,
                       device_cap=cap)
-        f64 = D.PCA.synthetic(spec, pf)
+        f64 = open_ctx(pf)
         f64.bench_pass(Q, 1)
         ms64 = f64.bench_pass(Q, a.fp64_steps)
         wall64 = ms64[4]
@@ -353,7 +372,7 @@ def main():
             # pass-saving schedule: (A A^T Q) X reused as the next first pass
             log("full PCA run, mixed with one pass saved per iteration")
             pm.strict_passes = 0
-            ctx = D.PCA.synthetic(spec, pm)
+            ctx = open_ctx(pm)
             s_sv, U_sv, rep = ctx.run()
             ctx.close()
             extra["mixed_saving_pca_wall_s"] = round(rep["wall"], 3)
