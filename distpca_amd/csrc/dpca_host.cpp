@@ -515,20 +515,6 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
             (void)hipEventRecord(c->ev_free[b], c->st);
             (void)hipEventRecord(c->ev_bounce[b], c->st);
         }
-        // The rest of the cap keeps the leading blocks in HBM once the first
-        // pass has copied them, so later passes stream only the others.
-        // Without the memory the context just streams every block.
-        {
-            const int64_t cb = c->chunk_rows * c->ldp;
-            const int64_t nblk = (c->Npad + c->chunk_rows - 1) / c->chunk_rows;
-            int64_t rb = std::max<int64_t>(0, std::min<int64_t>((c->p.device_cap - 2 * cb) / cb, nblk - 1));
-            if (rb > 0 && hipMalloc((void **)&c->resident, (size_t)(rb * cb)) != hipSuccess) {
-                (void)hipGetLastError();
-                c->resident = nullptr;
-                rb = 0;
-            }
-            c->res_blocks = rb;
-        }
         // Pinned host memory is limited by the driver (typically half the
         // RAM); above that the matrix stays pageable and each block is staged
         // through a pinned buffer.
@@ -568,6 +554,22 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
     if (!c->streamed && (r = dalloc(&c->packed, (size_t)bytes))) {
         free_ctx(c);
         return r;
+    }
+    if (c->streamed) {
+        // The rest of the cap keeps the leading blocks in HBM once the first
+        // pass has copied them, so later passes stream only the others.
+        // Allocated last, so it never takes memory the work buffers need;
+        // without the memory the context just streams every block.
+        const int64_t cb = c->chunk_rows * c->ldp;
+        const int64_t nblk = (c->Npad + c->chunk_rows - 1) / c->chunk_rows;
+        int64_t rb = std::max<int64_t>(
+            0, std::min<int64_t>((c->p.device_cap - 2 * cb) / cb, nblk - 1));
+        if (rb > 0 && hipMalloc((void **)&c->resident, (size_t)(rb * cb)) != hipSuccess) {
+            (void)hipGetLastError();
+            c->resident = nullptr;
+            rb = 0;
+        }
+        c->res_blocks = rb;
     }
     *out = c;
     return 0;
