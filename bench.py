@@ -60,6 +60,21 @@ def layout_bytes(M, N, host_mode):
     return -(-N // 1024) * 1024 * (-(-np_ // 256) * 256)
 
 
+class stdout_to_stderr:
+    """Sends what the library prints on fd 1 (its per-iteration lines at
+    verbose 2) to stderr while a full run is in progress, so the JSON line
+    stays the only thing on stdout and a long run shows its progress."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
 def max_principal_angle(U, V):
     """Largest principal angle (radians) between span(U) and span(V), from
     the sines (accurate for small angles)."""
@@ -287,7 +302,7 @@ def main():
         return D.PCA.synthetic(spec, params)
 
     pm = D.Params(nsv=nsv, nrhs=nrhs, power=power, bsize=bsize, precision=D.MIXED,
-                  device_cap=cap)
+                  device_cap=cap, verbose=2)
     mixed = open_ctx(pm)
     log(f"generated + stats in {time.perf_counter() - t0:.1f} s")
     Q = np.linalg.qr(D.omega(M, nrhs))[0]
@@ -344,12 +359,30 @@ def main():
                 "za_ms": round(ms[1], 3), "zt_ms": round(ms[2], 3), "aux_ms": round(ms[3], 3),
                 "hbm_frac_packed": round(packed_bytes / (ms[dom] * 1e-3) / PEAK_HBM, 4)}
 
+    # One context at a time after the mixed pass timing: HIP spreads streams
+    # over a few hardware queues, and a second streamed context's streams can
+    # share a queue with the first's copy stream, serialising its copies with
+    # its kernels.
     extra = {}
+    full = not a.no_fp64 and not a.no_full
+    if full:
+        log("full PCA run, mixed")
+        with stdout_to_stderr():
+            s_mx, U_mx, rep = mixed.run()
+        extra["mixed_pca_wall_s"] = round(rep["wall"], 3)
+        extra["mixed_pca_its"] = rep["total_its"]
+        extra["mixed_pca_passes"] = rep["passes"]
+        extra["mixed_sigma"] = [float(x) for x in s_mx]
+    cpu = None
+    if not a.no_cpu:
+        log("cpu baseline (oracle pass on a bounded sample)")
+        cpu = cpu_baseline(mixed, M, N, nrhs, a.cpu_seconds)
+    mixed.close()
+
     if not a.no_fp64:
-        mixed_handle = mixed
         log(f"fp64: {a.fp64_steps} passes")
         pf = D.Params(nsv=nsv, nrhs=nrhs, power=power, bsize=bsize, precision=D.FP64,
-                      device_cap=cap)
+                      device_cap=cap, verbose=2)
         f64 = open_ctx(pf)
         f64.bench_pass(Q, 1)
         ms64 = f64.bench_pass(Q, a.fp64_steps)
@@ -359,21 +392,22 @@ def main():
         extra["fp64_zt_ms"] = round(ms64[2], 3)
         extra["fp64_frac_of_peak"] = round(2 * flops_kernel / (wall64 * 1e-3) / PEAK_FP64, 4)
         extra["mixed_speedup_vs_fp64"] = round(wall64 / ms_step, 3)
-        if not a.no_full:
-            for name, ctx in (("mixed", mixed_handle), ("fp64", f64)):
-                log(f"full PCA run, {name}")
-                s, U, rep = ctx.run()
-                extra[f"{name}_pca_wall_s"] = round(rep["wall"], 3)
-                extra[f"{name}_pca_its"] = rep["total_its"]
-                extra[f"{name}_pca_passes"] = rep["passes"]
-                extra[f"{name}_sigma"] = [float(x) for x in s]
-                if name == "mixed":
-                    s_mx, U_mx = s, U
+        if full:
+            log("full PCA run, fp64")
+            with stdout_to_stderr():
+                s, U, rep = f64.run()
+            extra["fp64_pca_wall_s"] = round(rep["wall"], 3)
+            extra["fp64_pca_its"] = rep["total_its"]
+            extra["fp64_pca_passes"] = rep["passes"]
+            extra["fp64_sigma"] = [float(x) for x in s]
+        f64.close()
+        if full:
             # pass-saving schedule: (A A^T Q) X reused as the next first pass
             log("full PCA run, mixed with one pass saved per iteration")
             pm.strict_passes = 0
             ctx = open_ctx(pm)
-            s_sv, U_sv, rep = ctx.run()
+            with stdout_to_stderr():
+                s_sv, U_sv, rep = ctx.run()
             ctx.close()
             extra["mixed_saving_pca_wall_s"] = round(rep["wall"], 3)
             extra["mixed_saving_pca_its"] = rep["total_its"]
@@ -385,13 +419,6 @@ def main():
                     j: max_principal_angle(U_x[:, :j], U[:, :j]) for j in (1, 4, 7, nsv)}
             extra["mixed_vs_fp64_eig_relerr"] = [
                 float(x) for x in np.abs(s_mx ** 2 - s ** 2) / s ** 2]
-        f64.close()
-
-    cpu = None
-    if not a.no_cpu:
-        log("cpu baseline (oracle pass on a bounded sample)")
-        cpu = cpu_baseline(mixed, M, N, nrhs, a.cpu_seconds)
-    mixed.close()
 
     out = {
         "metric": "PCA wall-s & HBM GB/s, mixed vs FP64 at equal PC error; p50/p99 serve latency",

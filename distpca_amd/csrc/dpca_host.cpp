@@ -168,6 +168,9 @@ struct dpca_ctx {
     float *Tpart = nullptr, *Apart = nullptr;
     int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 4;
     int64_t seg_bytes = 0, rps = 0;
+    int nseg64 = 1;                   // FP64 Z Q sample segments
+    int64_t seg64 = 0;
+    double *Tseg = nullptr;           // their partials (nseg64 > 1)
     // streaming from host memory or the .bed file (matrix above device_cap)
     bool streamed = false;
     int hmode = DPCA_HOST_PINNED;     // where the streamed matrix lives
@@ -269,7 +272,7 @@ static void free_ctx(dpca_ctx *c)
     (void)hipSetDevice(c->dev);
     void *ptrs[] = {c->packed, c->zval, c->mu, c->dsc, c->ssum, c->sctr, c->Q, c->ARHS,
                     c->RHS2, c->RHS2old, c->tmp, c->T, c->part, c->gpart, c->G, c->Rinv,
-                    c->w, c->X, c->dots, c->flags, c->cinfo, c->its, c->UW8, c->Q8, c->qmax, c->smax, c->Tpart, c->Apart};
+                    c->w, c->X, c->dots, c->flags, c->cinfo, c->its, c->UW8, c->Q8, c->qmax, c->smax, c->Tpart, c->Apart, c->Tseg};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->resident) (void)hipFree(c->resident);
@@ -379,13 +382,26 @@ static int alloc_work(dpca_ctx *c)
     int64_t sp = (2048 + tiles - 1) / tiles;
     sp = std::max<int64_t>(1, std::min<int64_t>(sp, B / 256));
     c->splits = (int)sp;
+    {
+        // FP64 Z Q (the FP64 form's passes, and the loadings of a model in
+        // either form): 128 SNP rows per workgroup; short blocks (streamed)
+        // also split the samples so that there are >= ~2048 workgroups
+        const int64_t rg = std::max<int64_t>(1, B / 128);
+        const int64_t ns = std::max<int64_t>(1, std::min<int64_t>((2048 + rg - 1) / rg, c->ldp / 64));
+        c->seg64 = round_up((c->ldp + ns - 1) / ns, 64);
+        c->nseg64 = (int)((c->ldp + c->seg64 - 1) / c->seg64);
+        if (c->nseg64 > 1) r |= dalloc(&c->Tseg, (size_t)c->nseg64 * B * c->kp * sizeof(double));
+    }
     if (c->p.precision == DPCA_FP64) {
         r |= dalloc(&c->part, (size_t)c->splits * mk);
     } else {
         // T = Z Q: sample segments of <= 65536 samples (the int32 bound of
         // the centred rows; fewer segments, fewer FP32 partials to reduce),
         // 2 or 1 SNP tiles per wave by column tiles.
-        const int64_t nsegs = (c->ldp + 16383) / 16384;
+        // (short blocks: enough segments for >= ~1024 workgroups)
+        const int64_t nsegs = std::min<int64_t>(
+            c->ldp / 128, std::max<int64_t>((c->ldp + 16383) / 16384,
+                                            (1024 + B / 256 - 1) / std::max<int64_t>(1, B / 256)));
         c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 128);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
         c->st_tiles = za_i8_tiles(c->k);
@@ -772,7 +788,8 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     if (c->p.precision == DPCA_FP64) {
         {
             Stage s(c, T_MM_A);
-            DPCA_CHECK(launch_za_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, Qin, c->kp, T));
+            DPCA_CHECK(launch_za_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, Qin, c->kp, T,
+                                     c->seg64, c->nseg64, c->Tseg));
         }
         if (half) return 0;
         {
@@ -1033,6 +1050,7 @@ static int run_once(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep, d
                     printf("Iteration %d: %d converged\n", ii, hs.conv);
                 else
                     printf("Iteration %d: MEV error: %02.13f\n", ii, hs.crit);
+                fflush(stdout);
             }
         }
         if (hs.stop) break;
@@ -1992,7 +2010,8 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
             r = half_pass(c, c->RHS2);
             c->p.precision = prec;
         } else {
-            e = launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->RHS2, c->kp, c->T);
+            e = launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->RHS2, c->kp, c->T,
+                              c->seg64, c->nseg64, c->Tseg);
         }
     }
     if (!r && e == hipSuccess) e = hipMemcpyAsync(inv, is2.data(), 64 * sizeof(double), hipMemcpyHostToDevice, c->st);

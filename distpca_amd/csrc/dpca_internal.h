@@ -66,9 +66,13 @@ int dense_top_svd(hipStream_t st, const double *B, int M, int64_t N, int nsv, do
 hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t rows,
                                int64_t ldp, int M, uint8_t *codes);
 
-// FP64 form, T = Z Q over all padded rows.  T: rows x kp.
+// FP64 form, T = Z Q over all padded rows.  T: rows x kp.  With nseg > 1 the
+// samples are split into nseg segments of seg_bytes packed bytes (more
+// workgroups for short blocks), partials in Tseg (nseg x rows x kp), summed
+// in segment order into T.
 hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                         const double *zval, const double *Q, int kp, double *T);
+                         const double *zval, const double *Q, int kp, double *T,
+                         int64_t seg_bytes, int nseg, double *Tseg);
 // FP64 form, partial[sp] = Z^T T over split sp of the rows.
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                          const double *zval, const double *T, int kp, int splits,

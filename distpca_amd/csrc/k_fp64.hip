@@ -298,11 +298,15 @@ hipError_t launch_decode_f64(hipStream_t st, const uint8_t *packed, int64_t rows
 // bytes, so every lane decodes from one 16-byte load per 256-sample chunk.
 // A workgroup owns 128 SNPs (4 waves x 2 tiles of 16) and the whole sample
 // range; Q (Mpad x kp, zero rows past M) is read through L1/L2.
+// blockIdx.y takes the packed bytes [y seg_bytes, (y + 1) seg_bytes) of every
+// row (a sample segment; one segment when the rows alone give enough
+// workgroups) and writes its partial T at T + y rows kp.
 template <int CT>
 __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ packed,
                                                 int64_t ldp, const double *__restrict__ zval,
                                                 const double *__restrict__ Q,
-                                                double *__restrict__ T)
+                                                double *__restrict__ T, int64_t seg_bytes,
+                                                int64_t rows)
 {
     constexpr int KP = CT * 16;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -321,7 +325,10 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
     }
     const uint8_t *p0 = packed + s0 * ldp + 16 * g;
     const uint8_t *p1 = packed + s1 * ldp + 16 * g;
-    for (int64_t cb = 0; cb < ldp; cb += 64) {
+    const int64_t cb0 = (int64_t)blockIdx.y * seg_bytes;
+    const int64_t cb1 = cb0 + seg_bytes < ldp ? cb0 + seg_bytes : ldp;
+    T += (int64_t)blockIdx.y * rows * KP;
+    for (int64_t cb = cb0; cb < cb1; cb += 64) {
         const uint4 v0 = *reinterpret_cast<const uint4 *>(p0 + cb);
         const uint4 v1 = *reinterpret_cast<const uint4 *>(p1 + cb);
         const uint32_t w0[4] = {v0.x, v0.y, v0.z, v0.w};
@@ -352,18 +359,25 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
 }
 
 hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                         const double *zval, const double *Q, int kp, double *T)
+                         const double *zval, const double *Q, int kp, double *T,
+                         int64_t seg_bytes, int nseg, double *Tseg)
 {
-    if (rows % 128 != 0) return hipErrorInvalidValue;
-    dim3 grid((unsigned)(rows / 128));
+    if (rows % 128 != 0 || seg_bytes % 64 != 0 || nseg < 1 || (nseg > 1 && !Tseg) ||
+        (int64_t)nseg * seg_bytes < ldp)
+        return hipErrorInvalidValue;
+    dim3 grid((unsigned)(rows / 128), (unsigned)nseg);
+    double *out = nseg > 1 ? Tseg : T;
     switch (kp) {
-    case 16: k_za_f64<1><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
-    case 32: k_za_f64<2><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
-    case 48: k_za_f64<3><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
-    case 64: k_za_f64<4><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, T); break;
+    case 16: k_za_f64<1><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
+    case 32: k_za_f64<2><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
+    case 48: k_za_f64<3><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
+    case 64: k_za_f64<4><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    // segments summed in order (deterministic)
+    if (e == hipSuccess && nseg > 1) e = launch_reduce_parts(st, Tseg, nseg, rows * kp, T, 0);
+    return e;
 }
 
 // ------------------------------------------------------- FP64  Z^T T part --
