@@ -413,6 +413,26 @@ static int alloc_work(dpca_ctx *c)
         const int64_t tl = c->ldp / (c->sa_tiles * 32);
         int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 32767) / 32768);
         ns = std::min<int64_t>(ns, B / 128);
+        {
+            // up to 4x more splits where they fill the last round of
+            // resident workgroups better: cost ~ rounds x (rows per split +
+            // 256 rows of per-workgroup start-up and write-out); taken only
+            // for a gain of > 3%, since each split adds a partial to reduce
+            int dev = 0, ncu = 0;
+            const int bpc = zt_i8_blocks_per_cu(c->k, c->sa_tiles);
+            if (bpc > 0 && hipGetDevice(&dev) == hipSuccess &&
+                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+                ncu > 0) {
+                const int64_t slots = (int64_t)bpc * ncu;
+                auto cost = [&](int64_t n) {
+                    return (tl * n + slots - 1) / slots * (round_up((B + n - 1) / n, 128) + 256);
+                };
+                int64_t best = ns;
+                for (int64_t n = ns + 1; n <= std::min<int64_t>(4 * ns, B / 128); n++)
+                    if (cost(n) < cost(best)) best = n;
+                if (cost(best) < 0.97 * cost(ns)) ns = best;
+            }
+        }
         c->rps = round_up((B + ns - 1) / ns, 128);
         c->nsplit = (int)((B + c->rps - 1) / c->rps);
         const int vt = col_tiles(c->k).vt();
@@ -567,6 +587,12 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
         free_ctx(c);
         return r;
     }
+    if (getenv("DPCA_SHOW_PLAN"))  // launch shapes, for tuning
+        fprintf(stderr,
+                "dpca plan: block rows %lld; fp64 splits %d, Z Q segments %d; int8 Z Q tiles %d "
+                "segments %d, Z^T T tiles %d splits %d (%lld rows)\n",
+                (long long)c->block_rows, c->splits, c->nseg64, c->st_tiles, c->nseg, c->sa_tiles,
+                c->nsplit, (long long)c->rps);
     if (!c->streamed && (r = dalloc(&c->packed, (size_t)bytes))) {
         free_ctx(c);
         return r;

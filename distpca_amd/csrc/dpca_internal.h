@@ -124,9 +124,13 @@ hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int
                         const double *dsc, int k, int st_tiles, int64_t seg_bytes, int nseg,
                         float *Tpart);
 // tiles per wave (sample tiles of k_zt_i8 for a batch of ldb bytes, SNP
-// tiles of k_za_i8) that keep the accumulators in registers
-inline int zt_i8_tiles(int k, int64_t ldb) { return (col_tiles(k).vt() <= 9 && ldb % 128 == 0) ? 4 : 2; }
-inline int za_i8_tiles(int k) { return col_tiles(k).vt() <= 9 ? 2 : 1; }
+// tiles of k_za_i8): the accumulators must fit the registers, and one tile
+// is taken where it doubles the workgroups resident per CU (needs the
+// current device; without one the two-tile form)
+int zt_i8_tiles(int k, int64_t ldb);
+int za_i8_tiles(int k);
+// workgroups of k_zt_i8 with sa_tiles resident per CU (0 if unknown)
+int zt_i8_blocks_per_cu(int k, int sa_tiles);
 
 // Tall-skinny helpers, FP64.  X, Y: m x kp row-major.
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,

@@ -15,6 +15,8 @@
 #include "dpca_internal.h"
 #include "k_common.h"
 
+#include <mutex>
+
 namespace dpca {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -669,6 +671,87 @@ static hipError_t za_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_
                                                  R, Tpart);
         return hipGetLastError();
     }
+}
+
+// Workgroups of each instance resident per CU, from the runtime (0 where it
+// cannot say, e.g. with no device), cached per column-tile shape.
+template <typename K>
+static int blocks_per_cu(K kern)
+{
+    int n = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 512, 0) == hipSuccess ? n : 0;
+}
+
+template <int SA, int F, int RT>
+static int zt_occ()
+{
+    if constexpr (SA * (3 * F + RT) > 36) return 0;
+    else return blocks_per_cu(k_zt_i8<SA, F, RT>);
+}
+
+template <int ST, int F, int RT>
+static int za_occ()
+{
+    if constexpr (ST * (3 * F + RT) > 18) return 0;
+    else return blocks_per_cu(k_za_i8<ST, F, RT, false>);
+}
+
+struct I8Occ {
+    int zt4, zt2, za2, za1;
+};
+
+static I8Occ i8_occ(int k)
+{
+    static std::mutex mu;
+    static int dev = -1;
+    static I8Occ cache[kMaxKp + 1];
+    static bool have[kMaxKp + 1];
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) return I8Occ{0, 0, 0, 0};
+    std::lock_guard<std::mutex> lk(mu);
+    if (d != dev) {
+        for (bool &h : have) h = false;
+        dev = d;
+    }
+    if (k < 1 || k > kMaxKp) return I8Occ{0, 0, 0, 0};
+    if (!have[k]) {
+        const ColTiles ct = col_tiles(k);
+        I8Occ o{0, 0, 0, 0};
+#define X(F_, RT_)                                                                          \
+    if (ct.F == F_ && ct.RT == RT_)                                                          \
+        o = I8Occ{zt_occ<4, F_, RT_>(), zt_occ<2, F_, RT_>(), za_occ<2, F_, RT_>(),           \
+                  za_occ<1, F_, RT_>()};
+        DPCA_I8_SHAPES(X)
+#undef X
+        // an unknown answer is not cached
+        if (o.zt4 <= 0 || o.zt2 <= 0 || o.za2 <= 0 || o.za1 <= 0) return o;
+        cache[k] = o;
+        have[k] = true;
+    }
+    return cache[k];
+}
+
+// Two tiles per wave reuse each operand word from LDS over twice the rows or
+// samples of one tile; one tile per wave (two sample tiles for Z^T T) is
+// chosen where it doubles the workgroups resident per CU instead.
+int zt_i8_tiles(int k, int64_t ldb)
+{
+    if (col_tiles(k).vt() > 9 || ldb % 128) return 2;
+    const I8Occ o = i8_occ(k);
+    return (o.zt4 > 0 && o.zt2 >= 2 * o.zt4) ? 2 : 4;
+}
+
+int za_i8_tiles(int k)
+{
+    if (col_tiles(k).vt() > 9) return 1;
+    const I8Occ o = i8_occ(k);
+    return (o.za2 > 0 && o.za1 >= 2 * o.za2) ? 1 : 2;
+}
+
+int zt_i8_blocks_per_cu(int k, int sa_tiles)
+{
+    const I8Occ o = i8_occ(k);
+    return sa_tiles == 4 ? o.zt4 : sa_tiles == 2 ? o.zt2 : 0;
 }
 
 // ldp must be a multiple of the tile's byte width (SA*32), rows and
