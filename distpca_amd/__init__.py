@@ -47,7 +47,7 @@ class Report(ctypes.Structure):
         ("wall", ctypes.c_double), ("last_crit", ctypes.c_double),
         ("t_h2d", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
         ("host_mode", ctypes.c_int), ("qr_repaired", ctypes.c_int),
-        ("resident_bytes", ctypes.c_double),
+        ("resident_bytes", ctypes.c_double), ("graph_its", ctypes.c_int),
     ]
 
     def as_dict(self):

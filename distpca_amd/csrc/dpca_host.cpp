@@ -139,6 +139,18 @@ struct Timer {
     hipEvent_t a, b;
 };
 enum { T_LOAD, T_MM_A, T_MM_AT, T_MM, T_GS, T_RR, T_OTHER, T_H2D, T_NKIND };
+// a graph-replayed iteration: its time is split over the kinds above in the
+// proportions measured on the eager first iteration
+constexpr int T_GRAPH = T_NKIND;
+
+// One steady-state outer iteration captured as a HIP graph.  The buffers Q,
+// tmp and ARHS trade places inside an iteration, so a graph is keyed by the
+// pointer state it starts from (`in`) and leaves (`out`).
+struct IterGraph {
+    double *in[3], *out[3];
+    hipGraphExec_t exec;
+    int passes;
+};
 
 struct dpca_ctx {
     dpca_params p{};
@@ -201,6 +213,12 @@ struct dpca_ctx {
     double t[T_NKIND] = {};
     double t_generate_rhs = 0;
     int passes = 0;
+    // iterations after the first replay captured graphs (resident, non-robust
+    // contexts)
+    std::vector<IterGraph> graphs;
+    bool capturing = false, graphs_off = false;
+    int graph_its = 0;                // iterations replayed in the last run
+    double iter_share[T_NKIND] = {};
 };
 
 static hipEvent_t get_event(dpca_ctx *c)
@@ -222,12 +240,15 @@ struct Stage {
     Stage(dpca_ctx *ctx, int kind, hipStream_t on = nullptr) : c(ctx), s(on ? on : ctx->st)
     {
         tm.kind = kind;
+        tm.a = tm.b = nullptr;
+        if (c->capturing) return;  // a graph being captured is timed as a whole
         tm.a = get_event(c);
         tm.b = get_event(c);
         if (tm.a) (void)hipEventRecord(tm.a, s);
     }
     ~Stage()
     {
+        if (c->capturing) return;
         if (tm.b) (void)hipEventRecord(tm.b, s);
         c->pending.push_back(tm);
     }
@@ -238,8 +259,13 @@ static void collect_timers(dpca_ctx *c)
 {
     for (auto &tm : c->pending) {
         float ms = 0;
-        if (tm.a && tm.b && hipEventElapsedTime(&ms, tm.a, tm.b) == hipSuccess)
-            c->t[tm.kind] += ms * 1e-3;
+        if (tm.a && tm.b && hipEventElapsedTime(&ms, tm.a, tm.b) == hipSuccess) {
+            if (tm.kind == T_GRAPH) {
+                for (int i = 0; i < T_NKIND; i++) c->t[i] += ms * 1e-3 * c->iter_share[i];
+            } else {
+                c->t[tm.kind] += ms * 1e-3;
+            }
+        }
         if (tm.a) c->free_ev.push_back(tm.a);
         if (tm.b) c->free_ev.push_back(tm.b);
     }
@@ -297,6 +323,7 @@ static void free_ctx(dpca_ctx *c)
         if (tm.b) (void)hipEventDestroy(tm.b);
     }
     for (auto e : c->free_ev) (void)hipEventDestroy(e);
+    for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
 }
@@ -836,7 +863,9 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     }
     {
         Stage s(c, T_MM);
-        DPCA_CHECK(launch_tprep(c->st, c->Tpart, c->nseg, rows, c->kp, T));
+        // the U / W maxima per SNP split come with T (not for a half pass)
+        DPCA_CHECK(launch_tprep(c->st, c->Tpart, c->nseg, rows, c->kp, T, c->mu + r0,
+                                c->dsc + r0, c->rps, half ? nullptr : c->smax));
     }
     if (half) return 0;
     const int nsplit = (int)((rows + c->rps - 1) / c->rps);
@@ -844,7 +873,7 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     {
         Stage s(c, T_MM);
         DPCA_CHECK(launch_uw8(c->st, T, rows, c->k, c->kp, c->mu + r0, c->dsc + r0, c->rps,
-                              c->smax, UW8));
+                              c->smax, UW8, true));
     }
     {
         Stage s(c, T_MM_AT);
@@ -988,6 +1017,123 @@ static int copy_out_rhs(dpca_ctx *c, const double *dev, double *host)
 // then repeats it in robust mode.
 constexpr int kRerunRobust = 1;
 
+// One outer iteration up to the stopping test (its state in c->its), as in
+// SubspaceIteration_MPI (src/methods.cpp:52-192) for the in-core variant and
+// BlockSubspaceIter_MPI_OOC_double_buffering (src/methods.cpp:938-1063)
+// otherwise.  No host synchronisation: every iteration after the first
+// (ii > 0) enqueues the same work, so it can be captured as a graph.
+static int iter_body(dpca_ctx *c, int ii)
+{
+    const dpca_params &p = c->p;
+    int r;
+    if (c->incore) {
+        // `power` full passes with QR between them, then QR, the half pass
+        // Z Q and the Rayleigh-Ritz matrix (Z Q)^T (Z Q).
+        if ((r = do_pass(c, c->RHS2, c->ARHS))) return r;
+        for (int kk = 0; kk < p.power - 1; kk++) {
+            if ((r = orthonormalise(c, c->ARHS))) return r;
+            if ((r = do_pass(c, c->Q, c->ARHS))) return r;
+        }
+        if ((r = orthonormalise(c, c->ARHS))) return r;
+        if ((r = half_pass(c, c->Q))) return r;
+        {
+            Stage s(c, T_RR);
+            DPCA_CHECK(launch_gram(c->st, c->T, c->T, c->Npad, c->kp, c->gpart, c->ngparts,
+                                   c->G));
+        }
+    } else {
+        if (ii > 0 && !p.strict_passes) {
+            // A A^T (Q X) = (A A^T Q) X: the previous iteration's last
+            // pass, rotated by the Ritz vectors, replaces the first pass.
+            Stage s(c, T_MM);
+            DPCA_CHECK(launch_apply(c->st, c->ARHS, c->X, c->Mpad, c->kp, c->tmp));
+            std::swap(c->ARHS, c->tmp);
+        } else if ((r = do_pass(c, c->RHS2, c->ARHS))) {
+            return r;
+        }
+        for (int kk = 0; kk < p.power - 1; kk++) {
+            if ((r = orthonormalise(c, c->ARHS))) return r;
+            if ((r = do_pass(c, c->Q, c->ARHS))) return r;
+        }
+        if ((r = orthonormalise(c, c->ARHS))) return r;
+        if ((r = do_pass(c, c->Q, c->ARHS))) return r;
+        Stage s(c, T_RR);
+        DPCA_CHECK(launch_gram(c->st, c->Q, c->ARHS, c->Mpad, c->kp, c->gpart, c->ngparts,
+                               c->G));
+    }
+    Stage s(c, T_RR);
+    DPCA_CHECK(launch_syev(c->st, c->G, c->k, c->kp, c->w, c->X, c->flags + 2));
+    DPCA_CHECK(launch_apply(c->st, c->Q, c->X, c->Mpad, c->kp, c->RHS2));
+    if (p.crit == 2 && ii > 0)
+        DPCA_CHECK(launch_col_dots(c->st, c->RHS2old, c->RHS2, c->Mpad, c->kp, c->nsv, c->gpart,
+                                   c->ngparts, c->dots));
+    DPCA_CHECK(launch_crit(c->st, c->w, c->nsv, p.crit, p.tol, ii, c->dots, c->its));
+    return 0;
+}
+
+// Graphs replay iterations of resident contexts; streamed passes are driven
+// from the host and the robust orthonormalisation synchronises, so those run
+// eagerly, as does everything with DPCA_NO_GRAPH set.
+static bool use_graphs(const dpca_ctx *c)
+{
+    return !c->streamed && !c->robust && !c->graphs_off && !getenv("DPCA_NO_GRAPH");
+}
+
+// Iteration ii > 0 from the graph for the current (Q, tmp, ARHS) state,
+// captured on first use.  The launches, their order and their arguments are
+// those iter_body enqueues eagerly, so the results are the same bit for bit.
+static int iter_replay(dpca_ctx *c, int ii)
+{
+    IterGraph *g = nullptr;
+    for (auto &x : c->graphs)
+        if (x.in[0] == c->Q && x.in[1] == c->tmp && x.in[2] == c->ARHS) g = &x;
+    if (!g) {
+        IterGraph ng{};
+        ng.in[0] = c->Q;
+        ng.in[1] = c->tmp;
+        ng.in[2] = c->ARHS;
+        const int passes0 = c->passes;
+        hipError_t e = hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal);
+        int r = 0;
+        hipGraph_t graph = nullptr;
+        if (e == hipSuccess) {
+            c->capturing = true;
+            r = iter_body(c, ii);
+            c->capturing = false;
+            e = hipStreamEndCapture(c->st, &graph);
+        }
+        if (e == hipSuccess && !r) e = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        ng.out[0] = c->Q;
+        ng.out[1] = c->tmp;
+        ng.out[2] = c->ARHS;
+        ng.passes = c->passes - passes0;
+        // nothing ran yet: back to the state the graph starts from
+        c->passes = passes0;
+        c->Q = ng.in[0];
+        c->tmp = ng.in[1];
+        c->ARHS = ng.in[2];
+        if (e != hipSuccess || r) {
+            // capture not available here: this and later iterations run eagerly
+            (void)hipGetLastError();
+            c->graphs_off = true;
+            return iter_body(c, ii);
+        }
+        c->graphs.push_back(ng);
+        g = &c->graphs.back();
+    }
+    {
+        Stage s(c, T_GRAPH);
+        DPCA_CHECK(hipGraphLaunch(g->exec, c->st));
+    }
+    c->Q = g->out[0];
+    c->tmp = g->out[1];
+    c->ARHS = g->out[2];
+    c->passes += g->passes;
+    c->graph_its++;
+    return 0;
+}
+
 static int run_once(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep, double t0,
                     bool fresh)
 {
@@ -999,6 +1145,7 @@ static int run_once(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep, d
         c->qr_repairs = 0;
     }
     c->crit_hist.clear();
+    c->graph_its = 0;
     std::vector<double> om;
     double *rhs_in = rhs;
     if (!rhs_in) {
@@ -1018,55 +1165,22 @@ static int run_once(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep, d
     IterState hs{};
     int ii = 0;
     for (ii = 0; ii < p.miter; ii++) {
-        if (c->incore) {
-            // SubspaceIteration_MPI (src/methods.cpp:52-192): `power` full
-            // passes with QR between them, then QR, the half pass Z Q and the
-            // Rayleigh-Ritz matrix (Z Q)^T (Z Q).
-            if ((r = do_pass(c, c->RHS2, c->ARHS))) return r;
-            for (int kk = 0; kk < p.power - 1; kk++) {
-                if ((r = orthonormalise(c, c->ARHS))) return r;
-                if ((r = do_pass(c, c->Q, c->ARHS))) return r;
-            }
-            if ((r = orthonormalise(c, c->ARHS))) return r;
-            if ((r = half_pass(c, c->Q))) return r;
-            {
-                Stage s(c, T_RR);
-                DPCA_CHECK(launch_gram(c->st, c->T, c->T, c->Npad, c->kp, c->gpart, c->ngparts,
-                                       c->G));
-            }
-        } else {
-            if (ii > 0 && !p.strict_passes) {
-                // A A^T (Q X) = (A A^T Q) X: the previous iteration's last
-                // pass, rotated by the Ritz vectors, replaces the first pass.
-                Stage s(c, T_MM);
-                DPCA_CHECK(launch_apply(c->st, c->ARHS, c->X, c->Mpad, c->kp, c->tmp));
-                std::swap(c->ARHS, c->tmp);
-            } else if ((r = do_pass(c, c->RHS2, c->ARHS))) {
-                return r;
-            }
-            for (int kk = 0; kk < p.power - 1; kk++) {
-                if ((r = orthonormalise(c, c->ARHS))) return r;
-                if ((r = do_pass(c, c->Q, c->ARHS))) return r;
-            }
-            if ((r = orthonormalise(c, c->ARHS))) return r;
-            if ((r = do_pass(c, c->Q, c->ARHS))) return r;
-            Stage s(c, T_RR);
-            DPCA_CHECK(launch_gram(c->st, c->Q, c->ARHS, c->Mpad, c->kp, c->gpart, c->ngparts,
-                                   c->G));
-        }
-        {
-            Stage s(c, T_RR);
-            DPCA_CHECK(launch_syev(c->st, c->G, c->k, c->kp, c->w, c->X, c->flags + 2));
-            DPCA_CHECK(launch_apply(c->st, c->Q, c->X, c->Mpad, c->kp, c->RHS2));
-            if (p.crit == 2 && ii > 0)
-                DPCA_CHECK(launch_col_dots(c->st, c->RHS2old, c->RHS2, c->Mpad, c->kp, c->nsv,
-                                           c->gpart, c->ngparts, c->dots));
-            DPCA_CHECK(launch_crit(c->st, c->w, c->nsv, p.crit, p.tol, ii, c->dots, c->its));
-            DPCA_CHECK(hipMemcpyAsync(&hs, c->its, sizeof(IterState), hipMemcpyDeviceToHost,
-                                      c->st));
-        }
+        double t_before[T_NKIND];
+        for (int i = 0; i < T_NKIND; i++) t_before[i] = c->t[i];
+        if (ii > 0 && use_graphs(c))
+            r = iter_replay(c, ii);
+        else
+            r = iter_body(c, ii);
+        if (r) return r;
+        DPCA_CHECK(hipMemcpyAsync(&hs, c->its, sizeof(IterState), hipMemcpyDeviceToHost, c->st));
         DPCA_CHECK(hipStreamSynchronize(c->st));
         collect_timers(c);
+        if (ii == 0) {
+            double sum = 0;
+            for (int i = 0; i < T_NKIND; i++) sum += c->t[i] - t_before[i];
+            for (int i = 0; i < T_NKIND; i++)
+                c->iter_share[i] = sum > 0 ? (c->t[i] - t_before[i]) / sum : (i == T_MM ? 1.0 : 0.0);
+        }
         if (ii > 0) {
             c->crit_hist.push_back(hs.crit);
             if (p.verbose > 1) {
@@ -1131,6 +1245,7 @@ static int run_once(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep, d
         rep->host_mode = c->streamed ? c->hmode : -1;
         rep->qr_repaired = c->qr_repairs;
         rep->resident_bytes = c->streamed ? (double)(c->res_blocks * c->chunk_rows * c->ldp) : 0.0;
+        rep->graph_its = c->graph_its;
     }
     return 0;
 }
@@ -2046,7 +2161,7 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->mu, c->mu, n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->dsc, c->dsc, n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess && m->UW8)
-        e = launch_uw8(c->st, m->W, c->Npad, c->k, c->kp, m->mu, m->dsc, prps, m->smax, m->UW8);
+        e = launch_uw8(c->st, m->W, c->Npad, c->k, c->kp, m->mu, m->dsc, prps, m->smax, m->UW8, false);
     if (!r && e == hipSuccess) e = hipStreamSynchronize(c->st);
     collect_timers(c);
     (void)hipFree(inv);

@@ -77,14 +77,24 @@ hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, in
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                          const double *zval, const double *T, int kp, int splits,
                          double *part);
+// out[e] (+)= sum over p < nparts of part[p * n + e] in FP64, in an order
+// fixed by (nparts, n, e) alone.
+hipError_t launch_sum_parts_f64(hipStream_t st, const double *part, int nparts, int64_t n,
+                                double *out, int accumulate);
+hipError_t launch_sum_parts_f32(hipStream_t st, const float *part, int nparts, int64_t n,
+                                double *out, int accumulate);
 // out (+)= sum_sp part[sp]; n = Mpad*kp
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
                                double *out, int accumulate);
 
 // Mixed form: exact int8 codes x three int8 slices of the FP64 operand,
 // int32 accumulation; see k_mixed.hip.
+// T = sum of the nseg FP32 partials of T (rows a multiple of 128); with
+// smax non-null also the per-split maxima that launch_uw8 needs (splits of
+// rps rows), so that it can be called with have_max.
 hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t rows, int kp,
-                        double *T);
+                        double *T, const double *mu, const double *dsc, int64_t rps,
+                        unsigned *smax);
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
                              double *out, int accumulate);
 // Column tiles of the int8 forms: the three slices of k right-hand columns
@@ -108,9 +118,11 @@ inline ColTiles col_tiles(int k)
 }
 // Z^T T on int8 MFMA: U = d T, W = -d mu T as three int8 slices per (SNP
 // split of rps rows, column) with a power-of-two scale from smax.
-// UW8: rows/8 x vt x 16 words of 16 bytes; smax: ceil(rows/rps) x kp.
+// UW8: rows/8 x vt x 16 words of 16 bytes; smax: ceil(rows/rps) x kp
+// (computed here unless have_max: then already computed by launch_tprep).
 hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int kp,
-                      const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8);
+                      const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8,
+                      bool have_max);
 hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                         const void *UW8, const unsigned *smax, int k, int sa_tiles,
                         int64_t rows_per_split, int nsplit, float *part);

@@ -463,23 +463,10 @@ hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, in
 }
 
 // out[e] (+)= sum over splits, in split order (deterministic).
-__global__ void k_reduce_parts(const double *__restrict__ part, int splits, int64_t n,
-                               double *__restrict__ out, int accumulate)
-{
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    double s = accumulate ? out[e] : 0.0;
-    for (int p = 0; p < splits; p++) s += part[(int64_t)p * n + e];
-    out[e] = s;
-}
-
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
                                double *out, int accumulate)
 {
-    if (n == 0) return hipSuccess;
-    k_reduce_parts<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, n, out,
-                                                                accumulate);
-    return hipGetLastError();
+    return launch_sum_parts_f64(st, part, splits, n, out, accumulate);
 }
 
 }  // namespace dpca

@@ -24,40 +24,65 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // ----------------------------------------------------- T segments -> T -----
 // T[s][c] = sum of the sample-segment partials of Z Q, in FP64.
-__global__ void k_tprep(const float *__restrict__ Tpart, int nseg, int64_t rows, int kp,
-                        double *__restrict__ T)
+// T = the sum of the segment partials (FP64, segment order); with smax,
+// also smax[split * kp + c] = max over the split's rows of f_s |T_sc| as
+// k_split_max (below) forms it, in the same pass over T.
+// Block: 4 x kp threads over rb rows (rb divides 128, so a block stays
+// inside one split: rps is a multiple of 128).
+__global__ void k_tprep(const float *__restrict__ Tpart, int nseg, int64_t rows, int kp, int rb,
+                        const double *__restrict__ mu, const double *__restrict__ dsc,
+                        int64_t rps, unsigned *__restrict__ smax, double *__restrict__ T)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * kp) return;
-    double t = 0;
-    for (int p = 0; p < nseg; p++) t += (double)Tpart[(int64_t)p * rows * kp + idx];
-    T[idx] = t;
+    __shared__ float red[256];
+    const int tid = threadIdx.x, rl = tid / kp, c = tid % kp;
+    const int64_t r0 = (int64_t)blockIdx.x * rb;
+    double m = 0;
+#pragma unroll 4
+    for (int64_t s = r0 + rl; s < r0 + rb; s += 4) {
+        const int64_t idx = s * kp + c;
+        double t = 0;
+        for (int p = 0; p < nseg; p++) t += (double)Tpart[(int64_t)p * rows * kp + idx];
+        T[idx] = t;
+        if (smax) {
+            const double f = fabs(dsc[s]) * fmax(1.0, fabs(mu[s]));
+            m = fmax(m, f * fabs(t));
+        }
+    }
+    if (!smax) return;  // the same for the whole block
+    float f = (float)m;
+    if ((double)f < m) f = __uint_as_float(__float_as_uint(f) + 1u);  // round up
+    red[tid] = f;
+    __syncthreads();
+    if (rl == 0) {
+        for (int j = 1; j < 4; j++) f = fmaxf(f, red[j * kp + c]);
+        if (f > 0) atomicMax(&smax[(r0 / rps) * kp + c], __float_as_uint(f));
+    }
 }
 
 hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t rows, int kp,
-                        double *T)
+                        double *T, const double *mu, const double *dsc, int64_t rps,
+                        unsigned *smax)
 {
-    const int64_t n = rows * kp;
-    k_tprep<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Tpart, nseg, rows, kp, T);
+    if (rows % 128 || kp * 4 > 256 || (smax && (rps % 128 || rps < 128)))
+        return hipErrorInvalidValue;
+    if (smax) {
+        // a fill kernel, not a memset node: safe inside a captured graph
+        const int64_t nsplit = (rows + rps - 1) / rps;
+        hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(smax), nsplit * kp * 4, 0);
+        if (e != hipSuccess) return e;
+    }
+    // short blocks of rows: >= ~2048 workgroups where the rows allow
+    int rb = 128;
+    while (rb > 16 && rows / rb < 2048) rb /= 2;
+    k_tprep<<<(unsigned)(rows / rb), 4 * kp, 0, st>>>(Tpart, nseg, rows, kp, rb, mu, dsc, rps, smax,
+                                                       T);
     return hipGetLastError();
-}
-
-// out[e] (+)= sum over splits of float partials, in FP64, split order.
-__global__ void k_reduce_f32(const float *__restrict__ part, int splits, int64_t n,
-                             double *__restrict__ out, int accumulate)
-{
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    double s = accumulate ? out[e] : 0.0;
-    for (int p = 0; p < splits; p++) s += (double)part[(int64_t)p * n + e];
-    out[e] = s;
 }
 
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
                              double *out, int accumulate)
 {
-    k_reduce_f32<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(part, splits, n, out, accumulate);
-    return hipGetLastError();
+    return launch_sum_parts_f32(st, part, splits, n, out, accumulate);
 }
 
 // ============================================== int8 slices and their tiles ==
@@ -175,7 +200,7 @@ __device__ __forceinline__ double combine2(const i32x4 (&acc)[3 * F + RT], int o
 // smax[split * kp + c] = max over rows of the split of f_s |T_sc|, with
 // f_s = |d_s| max(1, |mu_s|) (covers |U| = |d T| and |W| = |d mu T|), or
 // f_s = 1 when dsc is null.
-// Block: 4 x kp threads over 1024 rows of one split; one atomic per column.
+// Block: 4 x kp threads over 128 rows of one split; one atomic per column.
 // smax must hold zeros (or smaller maxima) on entry.
 __global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
                             const double *__restrict__ mu, const double *__restrict__ dsc,
@@ -184,8 +209,8 @@ __global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
     __shared__ float red[256];
     const int tid = threadIdx.x, rl = tid / kp, c = tid % kp;
     const int64_t split = blockIdx.y;
-    const int64_t r0 = split * rps + (int64_t)blockIdx.x * 1024;
-    int64_t r1 = r0 + 1024;
+    const int64_t r0 = split * rps + (int64_t)blockIdx.x * 128;
+    int64_t r1 = r0 + 128;
     if (r1 > (split + 1) * rps) r1 = (split + 1) * rps;
     if (r1 > rows) r1 = rows;
     double m = 0;
@@ -244,17 +269,20 @@ __global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F,
 }
 
 hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int kp,
-                      const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8)
+                      const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8,
+                      bool have_max)
 {
     const ColTiles ct = col_tiles(k);
     if (rows % 128 || rps % 128 || rps < 128 || kp * 4 > 256 || ct.kp() != kp)
         return hipErrorInvalidValue;
     const int64_t nsplit = (rows + rps - 1) / rps;
-    // a fill kernel, not a memset node: safe inside a captured graph
-    hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(smax), nsplit * kp * 4, 0);
-    if (e != hipSuccess) return e;
-    dim3 g1((unsigned)((rps + 1023) / 1024), (unsigned)nsplit);
-    k_split_max<<<g1, 4 * kp, 0, st>>>(T, rows, kp, mu, dsc, rps, smax);
+    if (!have_max) {
+        // a fill kernel, not a memset node: safe inside a captured graph
+        hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(smax), nsplit * kp * 4, 0);
+        if (e != hipSuccess) return e;
+        dim3 g1((unsigned)((rps + 127) / 128), (unsigned)nsplit);
+        k_split_max<<<g1, 4 * kp, 0, st>>>(T, rows, kp, mu, dsc, rps, smax);
+    }
     const int64_t n = rows / 8 * ct.vt() * 16;
     k_uw8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc,
                                                         rps, smax, reinterpret_cast<u32x4 *>(UW8));
@@ -457,7 +485,7 @@ hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int k
     const int64_t nseg = (Mpad + segs - 1) / segs;
     hipError_t e = launch_fill_u8(st, reinterpret_cast<uint8_t *>(qmax), nseg * kp * 4, 0);
     if (e != hipSuccess) return e;
-    dim3 g1((unsigned)((segs + 1023) / 1024), (unsigned)nseg);
+    dim3 g1((unsigned)((segs + 127) / 128), (unsigned)nseg);
     k_split_max<<<g1, 4 * kp, 0, st>>>(Q, Mpad, kp, nullptr, nullptr, segs, qmax);
     const int64_t n = Mpad / 16 * ct.vt() * 16;
     k_q8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Q, Mpad, kp, ct.F, ct.R, ct.vt(), segs, qmax,

@@ -6,6 +6,8 @@
 #include "dpca_internal.h"
 #include "k_common.h"
 
+#include <algorithm>
+
 namespace dpca {
 
 // ------------------------------------------------------------------- Gram --
@@ -54,19 +56,66 @@ __global__ __launch_bounds__(256) void k_gram_part(const double *__restrict__ X,
     }
 }
 
-__global__ void k_sum_parts(const double *__restrict__ part, int nparts, int n,
-                            double *__restrict__ out)
+// out[e] (+)= sum over p < nparts of part[p * n + e], in FP64.  W waves per
+// workgroup (W from nparts only): wave w sums the partials p = w, w + W, ...
+// for 64 consecutive e, then the W sums are added in wave order, so the
+// result depends on (nparts, n, e) alone, never on timing or on the grid.
+template <typename T>
+__global__ __launch_bounds__(1024) void k_sum_parts(const T *__restrict__ part, int nparts,
+                                                    int64_t n, int W, double *__restrict__ out,
+                                                    int accumulate)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    __shared__ double red[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t e = (int64_t)blockIdx.x * 64 + lane;
     double s = 0;
-    for (int p = 0; p < nparts; p++) s += part[(int64_t)p * n + e];
-    out[e] = s;
+    if (e < n) {
+#pragma unroll 4
+        for (int p = w; p < nparts; p += W) s += (double)part[(int64_t)p * n + e];
+    }
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && e < n) {
+        double t = accumulate ? out[e] : 0.0;
+        for (int j = 0; j < W; j++) t += red[j][lane];
+        out[e] = t;
+    }
+}
+
+template <typename T>
+static hipError_t sum_parts(hipStream_t st, const T *part, int nparts, int64_t n, double *out,
+                            int accumulate)
+{
+    if (n <= 0) return hipSuccess;
+    // >= ~8 partials per wave, at most 16 waves
+    const int W = nparts >= 128 ? 16 : std::max(1, nparts / 8);
+    k_sum_parts<T><<<(unsigned)((n + 63) / 64), 64 * W, 0, st>>>(part, nparts, n, W, out,
+                                                                  accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_parts_f64(hipStream_t st, const double *part, int nparts, int64_t n,
+                                double *out, int accumulate)
+{
+    return sum_parts(st, part, nparts, n, out, accumulate);
+}
+
+hipError_t launch_sum_parts_f32(hipStream_t st, const float *part, int nparts, int64_t n,
+                                double *out, int accumulate)
+{
+    return sum_parts(st, part, nparts, n, out, accumulate);
+}
+
+// Row blocks of >= 32 rows (one LDS tile): small m uses fewer partials.
+static int gram_parts(int64_t m, int nparts)
+{
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nparts, (m + 31) / 32));
 }
 
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
                        double *part, int nparts, double *G)
 {
+    nparts = gram_parts(m, nparts);
     int64_t rpp = (m + nparts - 1) / nparts;
     switch (kp) {
     case 16: k_gram_part<16><<<nparts, 256, 0, st>>>(X, Y, m, rpp, part); break;
@@ -75,9 +124,9 @@ hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t
     case 64: k_gram_part<64><<<nparts, 256, 0, st>>>(X, Y, m, rpp, part); break;
     default: return hipErrorInvalidValue;
     }
-    int n = kp * kp;
-    k_sum_parts<<<(n + 255) / 256, 256, 0, st>>>(part, nparts, n, G);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sum_parts(st, (const double *)part, nparts, (int64_t)kp * kp, G, 0);
 }
 
 // -------------------------------------------------- Cholesky + R^{-1} ------
@@ -114,10 +163,10 @@ __device__ int chol_factor(double (*A)[kMaxKp + 1], const double *diag0, int k, 
         __syncthreads();
         const double piv = A[j][j];
         const bool dj = drop;
-        for (int c = j + 1 + tid; c < k; c += 256) A[j][c] = dj ? 0.0 : A[j][c] / piv;
+        for (int c = j + 1 + tid; c < k; c += blockDim.x) A[j][c] = dj ? 0.0 : A[j][c] / piv;
         __syncthreads();
         const int n = k - j - 1;
-        for (int e = tid; e < n * n; e += 256) {
+        for (int e = tid; e < n * n; e += blockDim.x) {
             const int r = j + 1 + e / n, c = j + 1 + e % n;
             if (r <= c) A[r][c] -= A[j][r] * A[j][c];
         }
@@ -146,7 +195,7 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
         bad = 0;
         skip = 0;
     }
-    for (int e = tid; e < kp * kp; e += 256) {
+    for (int e = tid; e < kp * kp; e += blockDim.x) {
         const int i = e / kp, j = e % kp;
         A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
     }
@@ -162,7 +211,7 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
     }
     __syncthreads();
     if (skip) {
-        for (int e = tid; e < kp * kp; e += 256)
+        for (int e = tid; e < kp * kp; e += blockDim.x)
             Rinv[e] = (e / kp == e % kp && e / kp < k) ? 1.0 : 0.0;
         return;
     }
@@ -177,7 +226,7 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
     }
     if (bad && mode == 0) {
         __syncthreads();
-        for (int e = tid; e < kp * kp; e += 256) {
+        for (int e = tid; e < kp * kp; e += blockDim.x) {
             const int i = e / kp, j = e % kp;
             A[i][j] = (i <= j) ? G[i * kp + j] : G[j * kp + i];
         }
@@ -210,7 +259,7 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
         if (tid == 0) A[j][j] = 1.0 / djj;
         __syncthreads();
     }
-    for (int e = tid; e < kp * kp; e += 256) {
+    for (int e = tid; e < kp * kp; e += blockDim.x) {
         const int i = e / kp, j = e % kp;
         Rinv[e] = (i < k && j < k && i <= j) ? A[i][j] : 0.0;
     }
@@ -221,6 +270,7 @@ hipError_t launch_chol_inv(hipStream_t st, const double *G, int k, int kp, doubl
                            int mode, double *Rinv, int *flags, double *cinfo)
 {
     if (kp > kMaxKp || k > kp) return hipErrorInvalidValue;
+    // 256 threads: one wave alone (cheaper barriers) measured 1.4x slower
     k_chol_inv<<<1, 256, 0, st>>>(G, k, kp, shift_mult, mode, Rinv, flags, cinfo);
     return hipGetLastError();
 }
@@ -321,7 +371,7 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ 
     __shared__ double red[256];
     __shared__ int done;
     const int tid = threadIdx.x;
-    for (int e = tid; e < k * k; e += 256) {
+    for (int e = tid; e < k * k; e += blockDim.x) {
         const int i = e / k, j = e % k;
         A[i][j] = (i <= j) ? B[i * kp + j] : B[j * kp + i];
         U[i][j] = (i == j) ? 1.0 : 0.0;
@@ -331,25 +381,31 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ 
     int sweeps = 0;
     __syncthreads();
     for (; sweeps < 40; sweeps++) {
-        // convergence: off-diagonal mass against the diagonal
-        double off = 0, dia = 0;
-        for (int e = tid; e < k * k; e += 256) {
-            const int i = e / k, j = e % k;
-            const double v = A[i][j] * A[i][j];
-            if (i == j) dia += v; else off += v;
+        // convergence: off-diagonal mass against the diagonal, summed as
+        // 256 strided partial sums and a fixed tree (the same additions
+        // whatever the block size)
+        double dia[256 / 64];  // blockDim.x >= 64
+        for (int vt = tid; vt < 256; vt += blockDim.x) {
+            double off = 0, dg = 0;
+            for (int e = vt; e < k * k; e += 256) {
+                const int i = e / k, j = e % k;
+                const double v = A[i][j] * A[i][j];
+                if (i == j) dg += v; else off += v;
+            }
+            red[vt] = off;
+            dia[vt / blockDim.x] = dg;
         }
-        red[tid] = off;
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {
-            if (tid < o) red[tid] += red[tid + o];
+            for (int vt = tid; vt < o; vt += blockDim.x) red[vt] += red[vt + o];
             __syncthreads();
         }
         const double offs = red[0];
         __syncthreads();
-        red[tid] = dia;
+        for (int vt = tid; vt < 256; vt += blockDim.x) red[vt] = dia[vt / blockDim.x];
         __syncthreads();
         for (int o = 128; o > 0; o >>= 1) {
-            if (tid < o) red[tid] += red[tid + o];
+            for (int vt = tid; vt < o; vt += blockDim.x) red[vt] += red[vt + o];
             __syncthreads();
         }
         if (tid == 0) done = !(offs > 1e-30 * red[0]);
@@ -384,7 +440,7 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ 
             }
             __syncthreads();
             // rows: A <- J^T A
-            for (int e = tid; e < half * k; e += 256) {
+            for (int e = tid; e < half * k; e += blockDim.x) {
                 const int pi = e / k, j = e % k;
                 const int p = rp[pi], q = rq[pi];
                 if (q < k) {
@@ -396,7 +452,7 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ 
             }
             __syncthreads();
             // columns: A <- A J,  U <- U J
-            for (int e = tid; e < half * k; e += 256) {
+            for (int e = tid; e < half * k; e += blockDim.x) {
                 const int pi = e / k, i = e % k;
                 const int p = rp[pi], q = rq[pi];
                 if (q < k) {
@@ -423,7 +479,7 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(const double *__restrict__ 
         w[rank] = d;
         for (int i = 0; i < kp; i++) V[i * kp + rank] = (i < k) ? U[i][tid] : 0.0;
     }
-    for (int e = tid; e < kp * kp; e += 256) {
+    for (int e = tid; e < kp * kp; e += blockDim.x) {
         const int i = e / kp, j = e % kp;
         if (j >= k) V[e] = 0.0;
         (void)i;
@@ -435,6 +491,7 @@ hipError_t launch_syev(hipStream_t st, const double *B, int k, int kp, double *w
                        int *status)
 {
     if (kp > kMaxKp || k > kp) return hipErrorInvalidValue;
+    // 256 threads: one wave alone (cheaper barriers) measured 2x slower
     k_syev_jacobi<<<1, 256, 0, st>>>(B, k, kp, w, V, status);
     return hipGetLastError();
 }
@@ -469,10 +526,12 @@ __global__ __launch_bounds__(256) void k_col_dots(const double *__restrict__ X,
 hipError_t launch_col_dots(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,
                            int ncols, double *part, int nparts, double *dots)
 {
+    nparts = gram_parts(m, nparts);
     int64_t rpp = (m + nparts - 1) / nparts;
     k_col_dots<<<nparts, 256, 0, st>>>(X, Y, m, kp, ncols, rpp, part);
-    k_sum_parts<<<(kp + 255) / 256, 256, 0, st>>>(part, nparts, kp, dots);
-    return hipGetLastError();
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sum_parts(st, (const double *)part, nparts, (int64_t)kp, dots, 0);
 }
 
 hipError_t launch_colsum(hipStream_t st, const double *X, int64_t m, int kp, double *part,

@@ -81,6 +81,10 @@ typedef struct dpca_report {
     int qr_repaired;        /* rank-deficient bases completed (k >= rank)     */
     double resident_bytes;  /* streamed: leading blocks kept in HBM after the
                                first pass (within device_cap)                 */
+    int graph_its;          /* outer iterations replayed from captured HIP
+                               graphs (resident contexts; the stage timers
+                               of those are split in the proportions of the
+                               eager first iteration)                        */
 } dpca_report;
 
 /* Defaults of initialize_structure (src/utilities.cpp:195-232). */

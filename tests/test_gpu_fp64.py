@@ -285,8 +285,13 @@ def test_cli_dropin(tmp_path):
 def test_rank_deficient_basis(incore):
     # nrhs = M: centred columns leave A A^T rank M - 1, so every basis the
     # iteration orthonormalises spans at most M - 1 dimensions; the reference's
-    # Householder QR completes it, the library replaces the dependent column
-    # and reports the completion
+    # Householder QR completes it with some orthonormal direction.  The
+    # library completes it too: the shifted CholeskyQR3 fallback turns the
+    # rounding noise of the dependent direction into one, or, where that
+    # fallback meets a non-positive pivot (which of the two happens depends
+    # on rounding at the 1e-16 level), the run is repeated with the dependent
+    # column replaced and the completion reported.  Either way the leading
+    # components are those of the oracle.
     _banner(f"rank-deficient basis (nrhs = M), incore {incore}")
     M, N = 30, 3000
     packed = _packed_random(M, N, seed=21)
@@ -295,9 +300,10 @@ def test_rank_deficient_basis(incore):
     ref = O.rsi(packed, M, N, O.resolve(p, M, N))
     with D.PCA.from_packed(packed, M, N, D.Params(**vars(p))) as pca:
         sigma, rhs, rep = pca.run()
-        assert rep["qr_repaired"] > 0
-        s2, _, rep2 = pca.run()          # robust from the start on this context
-        assert rep2["qr_repaired"] > 0 and np.array_equal(s2, sigma)
+        assert np.abs(rhs.T @ rhs - np.eye(30)).max() < 1e-10
+        if rep["qr_repaired"] > 0:
+            s2, _, rep2 = pca.run()          # robust from the start on this context
+            assert rep2["qr_repaired"] > 0 and np.array_equal(s2, sigma)
     assert rep["total_its"] == ref.total_its
     rel = np.abs(sigma[:9] ** 2 - ref.sigma[:9] ** 2) / ref.sigma[:9] ** 2
     assert rel.max() < 1e-10, rel
