@@ -1079,6 +1079,13 @@ static bool use_graphs(const dpca_ctx *c)
     return !c->streamed && !c->robust && !c->graphs_off && !getenv("DPCA_NO_GRAPH");
 }
 
+namespace dpca {
+void ctx_eager_iterations(dpca_ctx *c)
+{
+    if (c) c->graphs_off = true;
+}
+}  // namespace dpca
+
 // Iteration ii > 0 from the graph for the current (Q, tmp, ARHS) state,
 // captured on first use.  The launches, their order and their arguments are
 // those iter_body enqueues eagerly, so the results are the same bit for bit.

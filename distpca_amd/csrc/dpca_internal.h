@@ -15,9 +15,16 @@
         }                                                                         \
     } while (0)
 
+struct dpca_ctx;
+
 namespace dpca {
 
 void set_error(const char *fmt, ...);
+
+// Run every outer iteration of this context eagerly (no graph replay): the
+// serving loop's jobs, where a replayed iteration enters the hardware queue
+// as one block and delays the projections queued behind it.
+void ctx_eager_iterations(::dpca_ctx *c);
 
 // ---------------------------------------------------------------- layout --
 // Packed genotypes live in HBM SNP-major like the .bed file, but each row is
