@@ -274,6 +274,12 @@ static void collect_timers(dpca_ctx *c)
 
 // When set, dalloc only adds up sizes (footprint estimate of a context).
 static thread_local size_t *g_dry = nullptr;
+// Contexts opened by this thread are serving jobs (see dpca::set_job_contexts).
+static thread_local bool g_job_ctx = false;
+
+namespace dpca {
+void set_job_contexts(bool on) { g_job_ctx = on; }
+}  // namespace dpca
 
 template <class P>
 static int dalloc(P **p, size_t bytes)
@@ -431,11 +437,14 @@ static int alloc_work(dpca_ctx *c)
                                             (1024 + B / 256 - 1) / std::max<int64_t>(1, B / 256)));
         c->seg_bytes = round_up((c->ldp + nsegs - 1) / nsegs, 128);
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
-        c->st_tiles = za_i8_tiles(c->k);
+        // serving jobs keep the two-tile forms (fewer resident waves, room
+        // for the projections beside them)
+        const bool two = g_job_ctx && col_tiles(c->k).vt() <= 9;
+        c->st_tiles = two ? 2 : za_i8_tiles(c->k);
         if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 1, 2
         // Z^T T: SNP splits of <= 32768 rows (fewer partials), >= ~1024
         // workgroups
-        c->sa_tiles = zt_i8_tiles(c->k, c->ldp);
+        c->sa_tiles = (two && c->ldp % 128 == 0) ? 4 : zt_i8_tiles(c->k, c->ldp);
         if (const char *e = getenv("DPCA_ZT_TILES")) c->sa_tiles = atoi(e);  // tuning knob: 2, 4
         const int64_t tl = c->ldp / (c->sa_tiles * 32);
         int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 32767) / 32768);
@@ -446,7 +455,7 @@ static int alloc_work(dpca_ctx *c)
             // 256 rows of per-workgroup start-up and write-out); taken only
             // for a gain of > 3%, since each split adds a partial to reduce
             int dev = 0, ncu = 0;
-            const int bpc = zt_i8_blocks_per_cu(c->k, c->sa_tiles);
+            const int bpc = g_job_ctx ? 0 : zt_i8_blocks_per_cu(c->k, c->sa_tiles);
             if (bpc > 0 && hipGetDevice(&dev) == hipSuccess &&
                 hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
                 ncu > 0) {

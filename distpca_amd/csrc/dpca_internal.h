@@ -25,6 +25,12 @@ void set_error(const char *fmt, ...);
 // serving loop's jobs, where a replayed iteration enters the hardware queue
 // as one block and delays the projections queued behind it.
 void ctx_eager_iterations(::dpca_ctx *c);
+// Contexts this thread opens (and footprints it estimates) from now on are
+// serving jobs: the int8 products keep two tiles per wave and the minimum
+// split count, so fewer of their waves are resident and the projections
+// running beside them find room sooner (measured at 20 requests/s:
+// projection p99 34-36 ms against 40-49 ms, job latency unchanged).
+void set_job_contexts(bool on);
 
 // ---------------------------------------------------------------- layout --
 // Packed genotypes live in HBM SNP-major like the .bed file, but each row is

@@ -256,7 +256,9 @@ static void worker_loop(dpca_server *s)
             job = t;
         }
         dpca_ctx *c = nullptr;
+        set_job_contexts(true);
         int r = dpca_open_synthetic(&job.spec, &job.params, &c);
+        set_job_contexts(false);
         std::vector<double> sigma;
         if (!r) {
             // measured at 20 requests/s: graph-replayed job iterations left
@@ -468,7 +470,9 @@ int dpca_server_submit_job(dpca_server *s, const dpca_gen_spec *spec, const dpca
     dpca_params q = *p;
     q.device = s->p.device;
     long long fp = 0;
+    set_job_contexts(true);
     int r = dpca_footprint(spec->M, spec->N, &q, &fp);
+    set_job_contexts(false);
     if (r) return r;
     fp += (long long)spec->N * (long long)std::max(1, spec->npop) * 4 + (64ll << 20);  // generator tables, slack
     std::lock_guard<std::mutex> lk(s->mu);
