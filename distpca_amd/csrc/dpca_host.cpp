@@ -850,13 +850,13 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     if (c->p.precision == DPCA_FP64) {
         {
             Stage s(c, T_MM_A);
-            DPCA_CHECK(launch_za_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, Qin, c->kp, T,
+            DPCA_CHECK(launch_za_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, Qin, c->k, c->kp, T,
                                      c->seg64, c->nseg64, c->Tseg));
         }
         if (half) return 0;
         {
             Stage s(c, T_MM_AT);
-            DPCA_CHECK(launch_zt_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, T, c->kp,
+            DPCA_CHECK(launch_zt_f64(c->st, blk, rows, c->ldp, c->zval + 4 * r0, T, c->k, c->kp,
                                      c->splits, c->part));
         }
         Stage s(c, T_MM);
@@ -2167,7 +2167,7 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
             r = half_pass(c, c->RHS2);
             c->p.precision = prec;
         } else {
-            e = launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->RHS2, c->kp, c->T,
+            e = launch_za_f64(c->st, c->packed, c->Npad, c->ldp, c->zval, c->RHS2, c->k, c->kp, c->T,
                               c->seg64, c->nseg64, c->Tseg);
         }
     }

@@ -83,12 +83,13 @@ hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t ro
 // samples are split into nseg segments of seg_bytes packed bytes (more
 // workgroups for short blocks), partials in Tseg (nseg x rows x kp), summed
 // in segment order into T.
+// k columns are computed (kp = k rounded up to 16; the rest written as 0).
 hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                         const double *zval, const double *Q, int kp, double *T,
+                         const double *zval, const double *Q, int k, int kp, double *T,
                          int64_t seg_bytes, int nseg, double *Tseg);
-// FP64 form, partial[sp] = Z^T T over split sp of the rows.
+// FP64 form, partial[sp] = Z^T T over split sp of the rows (k columns as above).
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                         const double *zval, const double *T, int kp, int splits,
+                         const double *zval, const double *T, int k, int kp, int splits,
                          double *part);
 // out[e] (+)= sum over p < nparts of part[p * n + e] in FP64, in an order
 // fixed by (nparts, n, e) alone.

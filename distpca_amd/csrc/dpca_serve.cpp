@@ -60,7 +60,7 @@ hipError_t launch_projection(hipStream_t st, const dpca_model *m, const ProjPlan
     const int64_t n = 4 * pl.ldb * m->kp;
     hipError_t e;
     if (m->precision == DPCA_FP64) {
-        e = launch_zt_f64(st, batch, m->Npad, pl.ldb, m->zval, m->W, m->kp, pl.splits,
+        e = launch_zt_f64(st, batch, m->Npad, pl.ldb, m->zval, m->W, m->k, m->kp, pl.splits,
                           (double *)part);
         if (e == hipSuccess) e = launch_reduce_parts(st, (const double *)part, pl.splits, n, scores, 0);
     } else {

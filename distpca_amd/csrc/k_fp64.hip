@@ -301,14 +301,24 @@ hipError_t launch_decode_f64(hipStream_t st, const uint8_t *packed, int64_t rows
 // blockIdx.y takes the packed bytes [y seg_bytes, (y + 1) seg_bytes) of every
 // row (a sample segment; one segment when the rows alone give enough
 // workgroups) and writes its partial T at T + y rows kp.
-template <int CT>
+//
+// Columns: CT whole 16-column tiles, then NR groups of 4 columns on
+// v_mfma_f64_4x4x4f64 (4 blocks of D(4x4) += A(4x4) B(4x4); lane
+// 16k + 4m + i holds A_m[i][k], lane 16k + 4m + j holds B_m[k][j], lane
+// 16i + 4m + j receives D_m[i][j]; checked on the device).  With block m =
+// row group r / 4 the A operand is the same lane value as in the 16x16 tile
+// (row r, K slot g), every block takes the same 4 columns of Q, and a 4x4x4
+// costs 17 cycles against 64 for a 16x16x4 whose 12 extra columns would be
+// zero padding (nrhs 20 = 16 + 4, 40 = 2 x 16 + 8).  The padding columns
+// past the last group are written as zeros.
+template <int CT, int NR>
 __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ packed,
                                                 int64_t ldp, const double *__restrict__ zval,
                                                 const double *__restrict__ Q,
                                                 double *__restrict__ T, int64_t seg_bytes,
                                                 int64_t rows)
 {
-    constexpr int KP = CT * 16;
+    constexpr int KP = (CT + (NR > 0 ? 1 : 0)) * 16;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int r = lane & 15, g = lane >> 4;
     const int64_t base = (int64_t)blockIdx.x * 128 + wave * 32;
@@ -317,11 +327,17 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
                  a03 = zval[4 * s0 + 3];
     const double a10 = zval[4 * s1 + 0], a11 = zval[4 * s1 + 1], a12 = zval[4 * s1 + 2],
                  a13 = zval[4 * s1 + 3];
-    d4 acc0[CT], acc1[CT];
+    d4 acc0[CT > 0 ? CT : 1], acc1[CT > 0 ? CT : 1];
+    double rem0[NR > 0 ? NR : 1], rem1[NR > 0 ? NR : 1];
 #pragma unroll
     for (int c = 0; c < CT; c++) {
         acc0[c] = d4{0, 0, 0, 0};
         acc1[c] = d4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int n = 0; n < NR; n++) {
+        rem0[n] = 0.0;
+        rem1[n] = 0.0;
     }
     const uint8_t *p0 = packed + s0 * ldp + 16 * g;
     const uint8_t *p1 = packed + s1 * ldp + 16 * g;
@@ -334,11 +350,14 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
         const uint32_t w0[4] = {v0.x, v0.y, v0.z, v0.w};
         const uint32_t w1[4] = {v1.x, v1.y, v1.z, v1.w};
         const double *qrow = Q + (4 * cb + 64 * g) * KP + r;
+        const double *qrem = Q + (4 * cb + 64 * g) * KP + CT * 16 + (r & 3);
 #pragma unroll 8
         for (int t = 0; t < 64; t++) {
-            double b[CT];
+            double b[CT > 0 ? CT : 1], br[NR > 0 ? NR : 1];
 #pragma unroll
             for (int c = 0; c < CT; c++) b[c] = qrow[t * KP + c * 16];
+#pragma unroll
+            for (int n = 0; n < NR; n++) br[n] = qrem[t * KP + 4 * n];
             const uint32_t c0 = code_at(w0, t), c1 = code_at(w1, t);
             const double x0 = sel4(c0, a00, a01, a02, a03);
             const double x1 = sel4(c1, a10, a11, a12, a13);
@@ -346,6 +365,11 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
             for (int c = 0; c < CT; c++) {
                 acc0[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, b[c], acc0[c], 0, 0, 0);
                 acc1[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, b[c], acc1[c], 0, 0, 0);
+            }
+#pragma unroll
+            for (int n = 0; n < NR; n++) {
+                rem0[n] = __builtin_amdgcn_mfma_f64_4x4x4f64(x0, br[n], rem0[n], 0, 0, 0);
+                rem1[n] = __builtin_amdgcn_mfma_f64_4x4x4f64(x1, br[n], rem1[n], 0, 0, 0);
             }
         }
     }
@@ -356,24 +380,51 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
             T[(base + 4 * q + g) * KP + c * 16 + r] = acc0[c][q];
             T[(base + 16 + 4 * q + g) * KP + c * 16 + r] = acc1[c][q];
         }
+    if constexpr (NR > 0) {
+        // D_m[i][j] of this lane: row 4m + i = 4 (r / 4) + g, column r % 4
+        const int row = 4 * (r >> 2) + g;
+#pragma unroll
+        for (int n = 0; n < NR; n++) {
+            T[(base + row) * KP + CT * 16 + 4 * n + (r & 3)] = rem0[n];
+            T[(base + 16 + row) * KP + CT * 16 + 4 * n + (r & 3)] = rem1[n];
+        }
+        constexpr int PADC = KP - CT * 16 - 4 * NR;  // zero columns of the last tile
+        if constexpr (PADC > 0)
+            for (int e = lane; e < 32 * PADC; e += 64)
+                T[(base + e / PADC) * KP + CT * 16 + 4 * NR + e % PADC] = 0.0;
+    }
 }
 
+// Whole 16-column tiles and 4-column groups for k columns padded to kp:
+// R = k mod 16 <= 12 remaining columns take ceil(R / 4) groups, more a tile.
+static bool f64_tiles(int k, int kp, int &ct, int &nr)
+{
+    ct = k / 16;
+    const int R = k % 16;
+    nr = (R > 0 && R <= 12) ? (R + 3) / 4 : 0;
+    if (R > 12) ct++;
+    return k >= 1 && 16 * (ct + (nr > 0 ? 1 : 0)) == kp && kp <= 64;
+}
+
+#define DPCA_F64_SHAPES(X)                                                                  \
+    X(1, 0) X(2, 0) X(3, 0) X(4, 0) X(0, 1) X(0, 2) X(0, 3) X(1, 1) X(1, 2) X(1, 3) X(2, 1) \
+        X(2, 2) X(2, 3) X(3, 1) X(3, 2) X(3, 3)
+
 hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                         const double *zval, const double *Q, int kp, double *T,
+                         const double *zval, const double *Q, int k, int kp, double *T,
                          int64_t seg_bytes, int nseg, double *Tseg)
 {
+    int ct, nr;
     if (rows % 128 != 0 || seg_bytes % 64 != 0 || nseg < 1 || (nseg > 1 && !Tseg) ||
-        (int64_t)nseg * seg_bytes < ldp)
+        (int64_t)nseg * seg_bytes < ldp || !f64_tiles(k, kp, ct, nr))
         return hipErrorInvalidValue;
     dim3 grid((unsigned)(rows / 128), (unsigned)nseg);
     double *out = nseg > 1 ? Tseg : T;
-    switch (kp) {
-    case 16: k_za_f64<1><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
-    case 32: k_za_f64<2><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
-    case 48: k_za_f64<3><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
-    case 64: k_za_f64<4><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); break;
-    default: return hipErrorInvalidValue;
-    }
+#define X(C_, N_)                                                                           \
+    if (ct == C_ && nr == N_)                                                                \
+        k_za_f64<C_, N_><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows);
+    DPCA_F64_SHAPES(X)
+#undef X
     hipError_t e = hipGetLastError();
     // segments summed in order (deterministic)
     if (e == hipSuccess && nseg > 1) e = launch_reduce_parts(st, Tseg, nseg, rows * kp, T, 0);
@@ -384,15 +435,17 @@ hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, in
 // D(16 samples x 16 rhs) += A(16 samples x 4 SNPs) B(4 SNPs x 16 rhs).
 // A workgroup owns 256 samples (64 bytes of every row) and a contiguous SNP
 // range (split sp); each wave owns 64 samples = 4 tiles.  64-SNP chunks of
-// packed bytes, value tables and T rows are staged in LDS.
-template <int CT>
+// packed bytes, value tables and T rows are staged in LDS.  Columns as in
+// k_za_f64: CT whole tiles, NR 4-column groups on v_mfma_f64_4x4x4f64 (block
+// m = sample group r / 4), zero padding columns.
+template <int CT, int NR>
 __global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ packed,
                                                 int64_t rows, int64_t rows_per_split,
                                                 int64_t ldp, const double *__restrict__ zval,
                                                 const double *__restrict__ T,
                                                 double *__restrict__ part, int64_t Mpad)
 {
-    constexpr int KP = CT * 16;
+    constexpr int KP = (CT + (NR > 0 ? 1 : 0)) * 16;
     __shared__ uint8_t pk[64][64];
     __shared__ double zl[64][4];
     __shared__ double tl[64][KP];
@@ -403,11 +456,15 @@ __global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ pack
     const int64_t sbeg = sp * rows_per_split;
     int64_t send = sbeg + rows_per_split;
     if (send > rows) send = rows;
-    d4 acc[4][CT];
+    d4 acc[4][CT > 0 ? CT : 1];
+    double rem[4][NR > 0 ? NR : 1];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < 4; a++) {
 #pragma unroll
         for (int c = 0; c < CT; c++) acc[a][c] = d4{0, 0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < NR; n++) rem[a][n] = 0.0;
+    }
 
     for (int64_t s0 = sbeg; s0 < send; s0 += 64) {
         __syncthreads();
@@ -422,9 +479,11 @@ __global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ pack
 #pragma unroll 4
         for (int step = 0; step < 16; step++) {
             const int sl = 4 * step + g;
-            double b[CT];
+            double b[CT > 0 ? CT : 1], br[NR > 0 ? NR : 1];
 #pragma unroll
             for (int c = 0; c < CT; c++) b[c] = tl[sl][c * 16 + r];
+#pragma unroll
+            for (int n = 0; n < NR; n++) br[n] = tl[sl][CT * 16 + 4 * n + (r & 3)];
 #pragma unroll
             for (int a = 0; a < 4; a++) {
                 const int i = wave * 64 + a * 16 + r;
@@ -433,6 +492,9 @@ __global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ pack
 #pragma unroll
                 for (int c = 0; c < CT; c++)
                     acc[a][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, b[c], acc[a][c], 0, 0, 0);
+#pragma unroll
+                for (int n = 0; n < NR; n++)
+                    rem[a][n] = __builtin_amdgcn_mfma_f64_4x4x4f64(x, br[n], rem[a][n], 0, 0, 0);
             }
         }
     }
@@ -443,22 +505,37 @@ __global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ pack
         for (int c = 0; c < CT; c++)
 #pragma unroll
             for (int q = 0; q < 4; q++) out[(a * 16 + 4 * q + g) * KP + c * 16 + r] = acc[a][c][q];
+    if constexpr (NR > 0) {
+        // D_m[i][j] of this lane: sample 4m + i = 4 (r / 4) + g of the tile,
+        // column r % 4 of the group
+        const int row = 4 * (r >> 2) + g;
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int n = 0; n < NR; n++)
+                out[(a * 16 + row) * KP + CT * 16 + 4 * n + (r & 3)] = rem[a][n];
+        constexpr int PADC = KP - CT * 16 - 4 * NR;
+        if constexpr (PADC > 0)
+            for (int e = lane; e < 64 * PADC; e += 64)
+                out[(e / PADC) * KP + CT * 16 + 4 * NR + e % PADC] = 0.0;
+    }
 }
 
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                         const double *zval, const double *T, int kp, int splits, double *part)
+                         const double *zval, const double *T, int k, int kp, int splits,
+                         double *part)
 {
-    if (rows % 64 != 0 || ldp % 64 != 0 || splits < 1) return hipErrorInvalidValue;
+    int ct, nr;
+    if (rows % 64 != 0 || ldp % 64 != 0 || splits < 1 || !f64_tiles(k, kp, ct, nr))
+        return hipErrorInvalidValue;
     int64_t rps = round_up((rows + splits - 1) / splits, 64);
     dim3 grid((unsigned)(ldp / 64), (unsigned)splits);
     int64_t Mpad = 4 * ldp;
-    switch (kp) {
-    case 16: k_zt_f64<1><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
-    case 32: k_zt_f64<2><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
-    case 48: k_zt_f64<3><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
-    case 64: k_zt_f64<4><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad); break;
-    default: return hipErrorInvalidValue;
-    }
+#define X(C_, N_)                                                                           \
+    if (ct == C_ && nr == N_)                                                                \
+        k_zt_f64<C_, N_><<<grid, 256, 0, st>>>(packed, rows, rps, ldp, zval, T, part, Mpad);
+    DPCA_F64_SHAPES(X)
+#undef X
     return hipGetLastError();
 }
 
