@@ -97,6 +97,22 @@ def test_pass_matches_oracle(M, N, k):
     assert err < 1e-12, err
 
 
+@pytest.mark.parametrize("k", [1, 4, 5, 12, 13, 16, 17, 28, 29, 33, 44, 45, 52, 60, 64])
+def test_pass_column_groups(k):
+    # every split of k columns into whole 16-column tiles and 4-column groups
+    # (k mod 16 in 1..12 -> ceil(k mod 16 / 4) groups, 13..15 -> a tile)
+    M, N = 300, 1500
+    _banner(f"pass column groups k={k}")
+    packed = _packed_random(M, N, seed=100 + k)
+    Q = np.random.default_rng(100 + k).standard_normal((M, k))
+    ref = O.OOCPass(packed, M, N, 512)(Q)
+    with D.PCA.from_packed(packed, M, N, D.Params(nsv=1, nrhs=k, bsize=512)) as pca:
+        got = pca.pass_(Q)
+    assert got.shape == (M, k)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    assert err < 1e-12, err
+
+
 def test_orthonormalize_cholqr2():
     rng = np.random.default_rng(7)
     A = rng.standard_normal((5000, 40)) @ np.diag(np.logspace(0, 4, 40))
