@@ -394,6 +394,35 @@ static int resolve_params(dpca_ctx *c, const dpca_params *in)
     return 0;
 }
 
+// Workgroup slots of the device (resident workgroups per CU x CUs), 0 if
+// unknown (no device).
+static int64_t device_slots(int blocks_per_cu)
+{
+    int dev = 0, ncu = 0;
+    if (blocks_per_cu <= 0 || hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    return (int64_t)blocks_per_cu * ncu;
+}
+
+// SNP split count for a product kernel with `tiles` workgroups per split over
+// B rows: n in [lo, hi] minimising rounds x (rows per split + 256 rows of
+// per-workgroup start-up and write-out), rounds = ceil(tiles n / slots) of
+// resident workgroups; `dflt` unless that saves > 3% (each split adds a
+// partial to reduce) or when the slots are unknown.
+static int64_t fit_splits(int64_t tiles, int64_t B, int64_t align, int64_t dflt, int64_t lo,
+                          int64_t hi, int64_t slots)
+{
+    if (slots <= 0) return dflt;
+    auto cost = [&](int64_t n) {
+        return (tiles * n + slots - 1) / slots * (round_up((B + n - 1) / n, align) + 256);
+    };
+    int64_t best = dflt;
+    for (int64_t n = std::max<int64_t>(1, lo); n <= hi; n++)
+        if (cost(n) < cost(best)) best = n;
+    return cost(best) < 0.97 * cost(dflt) ? best : dflt;
+}
+
 static int alloc_work(dpca_ctx *c)
 {
     const size_t mk = (size_t)c->Mpad * c->kp * sizeof(double);
@@ -410,10 +439,14 @@ static int alloc_work(dpca_ctx *c)
     r |= dalloc(&c->RHS2old, mk);
     r |= dalloc(&c->tmp, mk);
     r |= dalloc(&c->T, (size_t)c->Npad * c->kp * sizeof(double));
-    // FP64 Z^T T: aim for >= ~2048 workgroups, each split >= 256 rows
+    // FP64 Z^T T: aim for >= ~2048 workgroups, each split >= 256 rows, then
+    // fit the count (half to twice that) to the resident workgroup slots
     const int64_t tiles = c->ldp / 64;
     int64_t sp = (2048 + tiles - 1) / tiles;
     sp = std::max<int64_t>(1, std::min<int64_t>(sp, B / 256));
+    if (!g_job_ctx)
+        sp = fit_splits(tiles, B, 64, sp, sp / 2, std::min<int64_t>(2 * sp, std::max<int64_t>(1, B / 256)),
+                        device_slots(zt_f64_blocks_per_cu(c->k, c->kp)));
     c->splits = (int)sp;
     {
         // FP64 Z Q (the FP64 form's passes, and the loadings of a model in
@@ -449,26 +482,11 @@ static int alloc_work(dpca_ctx *c)
         const int64_t tl = c->ldp / (c->sa_tiles * 32);
         int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 32767) / 32768);
         ns = std::min<int64_t>(ns, B / 128);
-        {
-            // up to 4x more splits where they fill the last round of
-            // resident workgroups better: cost ~ rounds x (rows per split +
-            // 256 rows of per-workgroup start-up and write-out); taken only
-            // for a gain of > 3%, since each split adds a partial to reduce
-            int dev = 0, ncu = 0;
-            const int bpc = g_job_ctx ? 0 : zt_i8_blocks_per_cu(c->k, c->sa_tiles);
-            if (bpc > 0 && hipGetDevice(&dev) == hipSuccess &&
-                hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-                ncu > 0) {
-                const int64_t slots = (int64_t)bpc * ncu;
-                auto cost = [&](int64_t n) {
-                    return (tl * n + slots - 1) / slots * (round_up((B + n - 1) / n, 128) + 256);
-                };
-                int64_t best = ns;
-                for (int64_t n = ns + 1; n <= std::min<int64_t>(4 * ns, B / 128); n++)
-                    if (cost(n) < cost(best)) best = n;
-                if (cost(best) < 0.97 * cost(ns)) ns = best;
-            }
-        }
+        // up to 4x more splits where they fill the last round of resident
+        // workgroups better
+        if (!g_job_ctx)
+            ns = fit_splits(tl, B, 128, ns, ns, std::min<int64_t>(4 * ns, B / 128),
+                            device_slots(zt_i8_blocks_per_cu(c->k, c->sa_tiles)));
         c->rps = round_up((B + ns - 1) / ns, 128);
         c->nsplit = (int)((B + c->rps - 1) / c->rps);
         const int vt = col_tiles(c->k).vt();

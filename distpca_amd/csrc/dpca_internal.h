@@ -91,6 +91,8 @@ hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, in
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                          const double *zval, const double *T, int k, int kp, int splits,
                          double *part);
+// workgroups of k_zt_f64 for k columns resident per CU (0 if unknown)
+int zt_f64_blocks_per_cu(int k, int kp);
 // out[e] (+)= sum over p < nparts of part[p * n + e] in FP64, in an order
 // fixed by (nparts, n, e) alone.
 hipError_t launch_sum_parts_f64(hipStream_t st, const double *part, int nparts, int64_t n,

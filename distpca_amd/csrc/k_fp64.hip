@@ -521,6 +521,18 @@ __global__ __launch_bounds__(256) void k_zt_f64(const uint8_t *__restrict__ pack
     }
 }
 
+int zt_f64_blocks_per_cu(int k, int kp)
+{
+    int ct, nr, n = 0;
+    if (!f64_tiles(k, kp, ct, nr)) return 0;
+    hipError_t e = hipErrorInvalidValue;
+#define X(C_, N_)                                                                           \
+    if (ct == C_ && nr == N_) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_zt_f64<C_, N_>, 256, 0);
+    DPCA_F64_SHAPES(X)
+#undef X
+    return e == hipSuccess ? n : 0;
+}
+
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                          const double *zval, const double *T, int k, int kp, int splits,
                          double *part)
