@@ -151,20 +151,19 @@ __device__ int chol_factor(double (*A)[kMaxKp + 1], const double *diag0, int k, 
                            int *colbad)
 {
     const int tid = threadIdx.x;
-    __shared__ int drop;
     for (int j = 0; j < k; j++) {
-        if (tid == 0) {
-            const double d = A[j][j];
-            drop = !(d > 0.0) || !(d > 1e-14 * diag0[j]);
-            if (drop) *bad = 1;
-            colbad[j] = drop;
-            A[j][j] = drop ? 1.0 : sqrt(d);
-        }
-        __syncthreads();
-        const double piv = A[j][j];
-        const bool dj = drop;
+        // every thread forms the pivot itself (two barriers per column);
+        // A[j][j] is written only in the update phase, which never reads it
+        const double d = A[j][j];
+        const bool dj = !(d > 0.0) || !(d > 1e-14 * diag0[j]);
+        const double piv = dj ? 1.0 : sqrt(d);
         for (int c = j + 1 + tid; c < k; c += blockDim.x) A[j][c] = dj ? 0.0 : A[j][c] / piv;
         __syncthreads();
+        if (tid == 0) {
+            if (dj) *bad = 1;
+            colbad[j] = dj;
+            A[j][j] = piv;
+        }
         const int n = k - j - 1;
         for (int e = tid; e < n * n; e += blockDim.x) {
             const int r = j + 1 + e / n, c = j + 1 + e % n;
@@ -245,17 +244,17 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double *__restrict__ G, 
         __syncthreads();
         chol_factor(A, diag0, k, &bad, colbad);
     }
-    // in-place inverse of the upper triangle, one column at a time
+    // in-place inverse of the upper triangle, one column at a time (column j
+    // and its diagonal are written only after every read of them)
     for (int j = 0; j < k; j++) {
+        const double djj = A[j][j];
         if (tid < j) {
             double s = 0;
             for (int l = tid; l < j; l++) s += A[tid][l] * A[l][j];
             col[tid] = s;
         }
         __syncthreads();
-        const double djj = A[j][j];
         if (tid < j) A[tid][j] = -col[tid] / djj;
-        __syncthreads();
         if (tid == 0) A[j][j] = 1.0 / djj;
         __syncthreads();
     }
