@@ -100,3 +100,23 @@ def test_cli_parameter_errors_before_any_device_call(tiny_bfile, args, msg):
     out = _cli(["-bfile", tiny_bfile] + args)
     assert out.returncode == 1
     assert msg in out.stdout, out.stdout
+
+
+def test_footprint_without_device():
+    # the HBM footprint of a context is host arithmetic (no device call): the
+    # resident 100k x 1M matrix plus its work buffers; above device_cap the
+    # two block buffers and the resident blocks stay within the cap
+    M, N = 100000, 1000000
+    packed = -(-N // 1024) * 1024 * (-(-((M + 3) // 4) // 256) * 256)
+    mixed = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.MIXED))
+    fp64 = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.FP64))
+    assert packed < mixed < packed + (3 << 30)
+    assert packed < fp64 < packed + (3 << 30)
+    cap = 2 * 10**9
+    streamed = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.MIXED,
+                                          device_cap=cap))
+    work = mixed - packed
+    assert streamed <= cap + work + (1 << 30)
+    assert streamed < mixed
+    # more SNPs, more bytes; the same matrix in either form holds the same packed bytes
+    assert D.footprint(M, 2 * N, D.Params(nsv=20, nrhs=40)) > D.footprint(M, N, D.Params(nsv=20, nrhs=40))
