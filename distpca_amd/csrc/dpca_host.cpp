@@ -450,9 +450,10 @@ static int alloc_work(dpca_ctx *c)
     c->splits = (int)sp;
     {
         // FP64 Z Q (the FP64 form's passes, and the loadings of a model in
-        // either form): 128 SNP rows per workgroup; short blocks (streamed)
-        // also split the samples so that there are >= ~2048 workgroups
-        const int64_t rg = std::max<int64_t>(1, B / 128);
+        // either form): 128 or 256 SNP rows per workgroup; short blocks
+        // (streamed) also split the samples so that there are >= ~2048
+        // workgroups
+        const int64_t rg = std::max<int64_t>(1, B / za_f64_rows_per_wg(c->k, c->kp));
         const int64_t ns = std::max<int64_t>(1, std::min<int64_t>((2048 + rg - 1) / rg, c->ldp / 64));
         c->seg64 = round_up((c->ldp + ns - 1) / ns, 64);
         c->nseg64 = (int)((c->ldp + c->seg64 - 1) / c->seg64);

@@ -87,6 +87,8 @@ hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t ro
 hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                          const double *zval, const double *Q, int k, int kp, double *T,
                          int64_t seg_bytes, int nseg, double *Tseg);
+// SNP rows one k_za_f64 workgroup covers (128 or 256)
+int za_f64_rows_per_wg(int k, int kp);
 // FP64 form, partial[sp] = Z^T T over split sp of the rows (k columns as above).
 hipError_t launch_zt_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                          const double *zval, const double *T, int k, int kp, int splits,

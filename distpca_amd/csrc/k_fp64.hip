@@ -8,6 +8,8 @@
 #include "dpca_internal.h"
 #include "k_common.h"
 
+#include <atomic>
+
 namespace dpca {
 
 // ------------------------------------------------------------------ repack --
@@ -296,8 +298,11 @@ hipError_t launch_decode_f64(hipStream_t st, const uint8_t *packed, int64_t rows
 // slot g is a sample.  Which sample sits in which K slot is free as long as
 // A and B agree: lane group g walks the 64 samples of its own 16 packed
 // bytes, so every lane decodes from one 16-byte load per 256-sample chunk.
-// A workgroup owns 128 SNPs (4 waves x 2 tiles of 16) and the whole sample
-// range; Q (Mpad x kp, zero rows past M) is read through L1/L2.
+// A workgroup owns 4 waves x 2 tiles of 16 SNPs (k_za_f64w: x 4 tiles, used
+// with two or more whole column tiles where it keeps two workgroups per CU:
+// each Q load then feeds twice the MFMAs; 5% faster at nrhs 40, 9% slower at
+// nrhs 20) and the whole sample range; Q (Mpad x kp, zero rows past M) is
+// read through L1/L2.
 // blockIdx.y takes the packed bytes [y seg_bytes, (y + 1) seg_bytes) of every
 // row (a sample segment; one segment when the rows alone give enough
 // workgroups) and writes its partial T at T + y rows kp.
@@ -395,6 +400,97 @@ __global__ __launch_bounds__(256) void k_za_f64(const uint8_t *__restrict__ pack
     }
 }
 
+// The same with ST SNP tiles per wave (used with ST = 4).
+template <int CT, int NR, int ST>
+__global__ __launch_bounds__(256) void k_za_f64w(const uint8_t *__restrict__ packed,
+                                                int64_t ldp, const double *__restrict__ zval,
+                                                const double *__restrict__ Q,
+                                                double *__restrict__ T, int64_t seg_bytes,
+                                                int64_t rows)
+{
+    constexpr int KP = (CT + (NR > 0 ? 1 : 0)) * 16;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int r = lane & 15, g = lane >> 4;
+    const int64_t base = (int64_t)blockIdx.x * (64 * ST) + wave * (16 * ST);
+    double av[ST][4];  // the 4-entry value table of this lane's SNP in each tile
+    const uint8_t *pp[ST];
+#pragma unroll
+    for (int s = 0; s < ST; s++) {
+        const int64_t row = base + 16 * s + r;
+#pragma unroll
+        for (int j = 0; j < 4; j++) av[s][j] = zval[4 * row + j];
+        pp[s] = packed + row * ldp + 16 * g;
+    }
+    d4 acc[ST][CT > 0 ? CT : 1];
+    double rem[ST][NR > 0 ? NR : 1];
+#pragma unroll
+    for (int s = 0; s < ST; s++) {
+#pragma unroll
+        for (int c = 0; c < CT; c++) acc[s][c] = d4{0, 0, 0, 0};
+#pragma unroll
+        for (int n = 0; n < NR; n++) rem[s][n] = 0.0;
+    }
+    const int64_t cb0 = (int64_t)blockIdx.y * seg_bytes;
+    const int64_t cb1 = cb0 + seg_bytes < ldp ? cb0 + seg_bytes : ldp;
+    T += (int64_t)blockIdx.y * rows * KP;
+    for (int64_t cb = cb0; cb < cb1; cb += 64) {
+        uint32_t w[ST][4];
+#pragma unroll
+        for (int s = 0; s < ST; s++) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(pp[s] + cb);
+            w[s][0] = v.x;
+            w[s][1] = v.y;
+            w[s][2] = v.z;
+            w[s][3] = v.w;
+        }
+        const double *qrow = Q + (4 * cb + 64 * g) * KP + r;
+        const double *qrem = Q + (4 * cb + 64 * g) * KP + CT * 16 + (r & 3);
+        constexpr int UNR = ST == 2 ? 8 : 4;
+#pragma unroll UNR
+        for (int t = 0; t < 64; t++) {
+            double b[CT > 0 ? CT : 1], br[NR > 0 ? NR : 1];
+#pragma unroll
+            for (int c = 0; c < CT; c++) b[c] = qrow[t * KP + c * 16];
+#pragma unroll
+            for (int n = 0; n < NR; n++) br[n] = qrem[t * KP + 4 * n];
+            double x[ST];
+#pragma unroll
+            for (int s = 0; s < ST; s++)
+                x[s] = sel4(code_at(w[s], t), av[s][0], av[s][1], av[s][2], av[s][3]);
+#pragma unroll
+            for (int c = 0; c < CT; c++)
+#pragma unroll
+                for (int s = 0; s < ST; s++)
+                    acc[s][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(x[s], b[c], acc[s][c], 0, 0, 0);
+#pragma unroll
+            for (int n = 0; n < NR; n++)
+#pragma unroll
+                for (int s = 0; s < ST; s++)
+                    rem[s][n] = __builtin_amdgcn_mfma_f64_4x4x4f64(x[s], br[n], rem[s][n], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < ST; s++)
+#pragma unroll
+        for (int c = 0; c < CT; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                T[(base + 16 * s + 4 * q + g) * KP + c * 16 + r] = acc[s][c][q];
+    if constexpr (NR > 0) {
+        // D_m[i][j] of this lane: row 4m + i = 4 (r / 4) + g, column r % 4
+        const int row = 4 * (r >> 2) + g;
+#pragma unroll
+        for (int s = 0; s < ST; s++)
+#pragma unroll
+            for (int n = 0; n < NR; n++)
+                T[(base + 16 * s + row) * KP + CT * 16 + 4 * n + (r & 3)] = rem[s][n];
+        constexpr int PADC = KP - CT * 16 - 4 * NR;  // zero columns of the last tile
+        if constexpr (PADC > 0)
+            for (int e = lane; e < 16 * ST * PADC; e += 64)
+                T[(base + e / PADC) * KP + CT * 16 + 4 * NR + e % PADC] = 0.0;
+    }
+}
+
 // Whole 16-column tiles and 4-column groups for k columns padded to kp:
 // R = k mod 16 <= 12 remaining columns take ceil(R / 4) groups, more a tile.
 static bool f64_tiles(int k, int kp, int &ct, int &nr)
@@ -410,19 +506,52 @@ static bool f64_tiles(int k, int kp, int &ct, int &nr)
     X(1, 0) X(2, 0) X(3, 0) X(4, 0) X(0, 1) X(0, 2) X(0, 3) X(1, 1) X(1, 2) X(1, 3) X(2, 1) \
         X(2, 2) X(2, 3) X(3, 1) X(3, 2) X(3, 3)
 
+// SNP tiles per wave of k_za_f64: 4 with two or more whole column tiles
+// where that form keeps >= 2 workgroups per CU (the runtime's occupancy,
+// cached per shape), else 2 (also without a device)
+static int za_f64_tiles(int ct, int nr)
+{
+    static std::atomic<int> cache[5][4];  // 0 unknown, else the tile count
+    if (ct < 2 || ct > 4 || nr < 0 || nr > 3) return 2;
+    const int have = cache[ct][nr].load(std::memory_order_relaxed);
+    if (have) return have;
+    int n = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define X(C_, N_)                                                                           \
+    if (ct == C_ && nr == N_) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_za_f64w<C_, N_, 4>, 256, 0);
+    DPCA_F64_SHAPES(X)
+#undef X
+    if (e != hipSuccess) return 2;
+    const int t = n >= 2 ? 4 : 2;
+    cache[ct][nr].store(t, std::memory_order_relaxed);
+    return t;
+}
+
+int za_f64_rows_per_wg(int k, int kp)
+{
+    int ct, nr;
+    return f64_tiles(k, kp, ct, nr) ? 64 * za_f64_tiles(ct, nr) : 128;
+}
+
 hipError_t launch_za_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                          const double *zval, const double *Q, int k, int kp, double *T,
                          int64_t seg_bytes, int nseg, double *Tseg)
 {
     int ct, nr;
-    if (rows % 128 != 0 || seg_bytes % 64 != 0 || nseg < 1 || (nseg > 1 && !Tseg) ||
-        (int64_t)nseg * seg_bytes < ldp || !f64_tiles(k, kp, ct, nr))
+    if (!f64_tiles(k, kp, ct, nr)) return hipErrorInvalidValue;
+    const int stl = za_f64_tiles(ct, nr);
+    if (rows % (64 * stl) != 0 || seg_bytes % 64 != 0 || nseg < 1 || (nseg > 1 && !Tseg) ||
+        (int64_t)nseg * seg_bytes < ldp)
         return hipErrorInvalidValue;
-    dim3 grid((unsigned)(rows / 128), (unsigned)nseg);
+    dim3 grid((unsigned)(rows / (64 * stl)), (unsigned)nseg);
     double *out = nseg > 1 ? Tseg : T;
 #define X(C_, N_)                                                                           \
-    if (ct == C_ && nr == N_)                                                                \
-        k_za_f64<C_, N_><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows);
+    if (ct == C_ && nr == N_) {                                                              \
+        if (stl == 4)                                                                        \
+            k_za_f64w<C_, N_, 4><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows); \
+        else                                                                                 \
+            k_za_f64<C_, N_><<<grid, 256, 0, st>>>(packed, ldp, zval, Q, out, seg_bytes, rows);    \
+    }
     DPCA_F64_SHAPES(X)
 #undef X
     hipError_t e = hipGetLastError();
