@@ -113,6 +113,25 @@ def test_pass_column_groups(k):
     assert err < 1e-12, err
 
 
+@pytest.mark.parametrize("k", [40, 64])
+def test_pass_wide_tiles_streamed(k):
+    # two or more whole column tiles take the 4-SNP-tile FP64 Z Q where it
+    # keeps two workgroups per CU; streamed 1024-row blocks also split the
+    # samples into segments (short blocks): both against the oracle
+    M, N = 2000, 6000
+    _banner(f"wide-tile pass k={k}, resident and streamed")
+    packed = _packed_random(M, N, seed=300 + k)
+    Q = np.random.default_rng(300 + k).standard_normal((M, k))
+    ref = O.OOCPass(packed, M, N, 512)(Q)
+    base = dict(nsv=2, nrhs=k, bsize=512)
+    with D.PCA.from_packed(packed, M, N, D.Params(**base)) as res, \
+            D.PCA.from_packed(packed, M, N, D.Params(device_cap=1 << 20, **base)) as stm:
+        a, b = res.pass_(Q), stm.pass_(Q)
+    for got in (a, b):
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < 1e-12, err
+
+
 def test_orthonormalize_cholqr2():
     rng = np.random.default_rng(7)
     A = rng.standard_normal((5000, 40)) @ np.diag(np.logspace(0, 4, 40))
