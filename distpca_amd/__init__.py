@@ -92,7 +92,7 @@ class FullSvdStats(ctypes.Structure):
 
 EXPORTS = [
     "dpca_params_init", "dpca_open", "dpca_open_packed", "dpca_open_synthetic",
-    "dpca_write_synthetic", "dpca_get_dims", "dpca_omega", "dpca_snp_stats", "dpca_decode",
+    "dpca_write_synthetic", "dpca_resolve", "dpca_get_dims", "dpca_omega", "dpca_snp_stats", "dpca_decode",
     "dpca_copy_packed", "dpca_pass", "dpca_run", "dpca_crit_history", "dpca_write_outputs",
     "dpca_orthonormalize", "dpca_syev", "dpca_bench_pass", "dpca_close", "dpca_last_error",
     "dpca_footprint", "dpca_model_from_run", "dpca_model_info", "dpca_model_project",
@@ -121,6 +121,7 @@ def lib():
     L.dpca_open_packed.argtypes = [vp, i32, i64, P(_Params), P(vp)]
     L.dpca_open_synthetic.argtypes = [P(GenSpec), P(_Params), P(vp)]
     L.dpca_write_synthetic.argtypes = [ctypes.c_char_p, P(GenSpec)]
+    L.dpca_resolve.argtypes = [i32, i64, P(_Params), P(_Params)]
     L.dpca_get_dims.argtypes = [vp, P(i32), P(i64), P(i32), P(i32), P(i32)]
     L.dpca_omega.argtypes = [i32, i32, vp]
     L.dpca_snp_stats.argtypes = [vp, vp, vp, vp]
@@ -196,6 +197,14 @@ class Params:
             setattr(p, f, getattr(self, f))
         p.prefix = self.prefix.encode()
         return p
+
+
+def resolve(M: int, N: int, params: Params) -> dict:
+    """bsize / nsv / nrhs / power as dpca_open resolves them (dpca_resolve,
+    src/distpca.cpp:239-352); host-only, no device is touched."""
+    p, out = params.c(), _Params()
+    _check(lib().dpca_resolve(M, N, ctypes.byref(p), ctypes.byref(out)))
+    return {f: getattr(out, f) for f in ("nsv", "nrhs", "power", "bsize")}
 
 
 def omega(M: int, k: int) -> np.ndarray:

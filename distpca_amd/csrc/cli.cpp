@@ -63,7 +63,9 @@ int ram_kb()
 // Console report of print_statistics (src/utilities.cpp:147-193).  The
 // scripts read field 5 of the first "Total wall-clock time elapsed:" line of
 // the seconds block; as in the reference it is the sum of the stage timers.
-void print_statistics(const dpca_report &r, int threads, int rkb, bool fullsvd)
+// The fetch count is the reference's total_its * (power + 1) under its own
+// schedule; under -strict 0 (strict == false) it is the passes the run made.
+void print_statistics(const dpca_report &r, int threads, int rkb, bool fullsvd, bool strict)
 {
     time_t now = time(nullptr);
     printf("----------------------------------------------------------\n");
@@ -79,10 +81,10 @@ void print_statistics(const dpca_report &r, int threads, int rkb, bool fullsvd)
     printf("Value of power: %d\n", r.power);
     printf("Total number of rows fetched from the disk per block: %d\n", r.bsize);
     if (r.bsize < r.N) {
+        const int fetches = strict ? r.total_its * (r.power + 1) : r.passes;
         printf("Total number of times which the data matrix was fetched from the memory: %d\n",
-               r.total_its * (r.power + 1));
-        printf("Average amount of time elapsed per matrix fetching: %lf\n",
-               r.t_load / (r.total_its * (r.power + 1)));
+               fetches);
+        printf("Average amount of time elapsed per matrix fetching: %lf\n", r.t_load / fetches);
     }
     const double tot = r.t_rr + r.t_generate_rhs + r.t_load + r.t_mm + r.t_gs + r.t_other;
     const double tot_noload = tot - r.t_load;
@@ -302,7 +304,7 @@ int main(int argc, char **argv)
     }
     if (p.verbose > 0) {
         const char *omp = getenv("OMP_NUM_THREADS");
-        print_statistics(rep, omp ? atoi(omp) : 1, rkb, p.fullsvd == 1);
+        print_statistics(rep, omp ? atoi(omp) : 1, rkb, p.fullsvd == 1, p.strict_passes != 0);
     }
     if (p.fwrite == 1 && dpca_write_outputs(ctx, p.prefix, rhs.data(), sigma.data()) != 0) {
         printf("\n%s\n", dpca_last_error());

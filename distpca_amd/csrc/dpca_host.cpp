@@ -334,6 +334,22 @@ static void free_ctx(dpca_ctx *c)
     delete c;
 }
 
+// Host RAM in KB: MemTotal of /proc/meminfo as GetRamInKB reads it
+// (src/utilities.cpp:60-80), as a 64-bit value (the reference's int
+// overflows above 2 TB); the page count where /proc is unreadable.
+static int64_t ram_kb()
+{
+    if (FILE *f = fopen("/proc/meminfo", "r")) {
+        char line[256];
+        long long kb = -1;
+        while (fgets(line, sizeof line, f))
+            if (sscanf(line, "MemTotal: %lld kB", &kb) == 1) break;
+        fclose(f);
+        if (kb > 0) return kb;
+    }
+    return (int64_t)sysconf(_SC_PHYS_PAGES) * (int64_t)sysconf(_SC_PAGE_SIZE) / 1024;
+}
+
 // bsize and NSV/NRHS clamps of src/distpca.cpp:218-352 for a single rank.
 static int resolve_params(dpca_ctx *c, const dpca_params *in)
 {
@@ -350,8 +366,7 @@ static int resolve_params(dpca_ctx *c, const dpca_params *in)
         return -1;
     }
     if (p.bsize <= 0) {
-        long pages = sysconf(_SC_PHYS_PAGES), psz = sysconf(_SC_PAGE_SIZE);
-        const double ram_gb = (double)pages * (double)psz / 1024.0 / 1e6;  // KB / 1e6
+        const double ram_gb = (double)ram_kb() / 1e6;  // KB / 1e6 (src/distpca.cpp:215)
         const double mem = (p.memory == 0.0) ? ram_gb / 10.0 : p.memory;
         const double membuff = 3.0 * N * 8.0 + (double)M * p.nsv * 8.0 + 2.0 * N * p.nrhs +
                                3.0 * M * 8.0 + 2048.0 * 100000.0;
@@ -475,11 +490,16 @@ static int alloc_work(dpca_ctx *c)
         // for the projections beside them)
         const bool two = g_job_ctx && col_tiles(c->k).vt() <= 9;
         c->st_tiles = two ? 2 : za_i8_tiles(c->k);
-        if (const char *e = getenv("DPCA_ZA_TILES")) c->st_tiles = atoi(e);  // tuning knob: 1, 2
+        // tuning knob: 1 or 2 (anything else keeps the computed choice)
+        if (const char *e = getenv("DPCA_ZA_TILES"))
+            if (!strcmp(e, "1") || !strcmp(e, "2")) c->st_tiles = atoi(e);
         // Z^T T: SNP splits of <= 32768 rows (fewer partials), >= ~1024
         // workgroups
         c->sa_tiles = (two && c->ldp % 128 == 0) ? 4 : zt_i8_tiles(c->k, c->ldp);
-        if (const char *e = getenv("DPCA_ZT_TILES")) c->sa_tiles = atoi(e);  // tuning knob: 2, 4
+        // tuning knob: 2 or 4, and 4 only where the row width is a multiple
+        // of its 128-byte tile (anything else keeps the computed choice)
+        if (const char *e = getenv("DPCA_ZT_TILES"))
+            if (!strcmp(e, "2") || (!strcmp(e, "4") && c->ldp % 128 == 0)) c->sa_tiles = atoi(e);
         const int64_t tl = c->ldp / (c->sa_tiles * 32);
         int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 32767) / 32768);
         ns = std::min<int64_t>(ns, B / 128);
@@ -644,9 +664,10 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
     }
     if (getenv("DPCA_SHOW_PLAN"))  // launch shapes, for tuning
         fprintf(stderr,
-                "dpca plan: block rows %lld; fp64 splits %d, Z Q segments %d; int8 Z Q tiles %d "
-                "segments %d, Z^T T tiles %d splits %d (%lld rows)\n",
-                (long long)c->block_rows, c->splits, c->nseg64, c->st_tiles, c->nseg, c->sa_tiles,
+                "dpca plan: block rows %lld; fp64 splits %d, Z Q segments %d, Z Q tiles %d; "
+                "int8 Z Q tiles %d segments %d, Z^T T tiles %d splits %d (%lld rows)\n",
+                (long long)c->block_rows, c->splits, c->nseg64,
+                za_f64_rows_per_wg(c->k, c->kp) / 64, c->st_tiles, c->nseg, c->sa_tiles,
                 c->nsplit, (long long)c->rps);
     if (!c->streamed && (r = dalloc(&c->packed, (size_t)bytes))) {
         free_ctx(c);
@@ -1636,6 +1657,21 @@ int dpca_write_synthetic(const char *prefix, const dpca_gen_spec *g)
     }
     fclose(f);
     gen_free(&tab);
+    return 0;
+}
+
+int dpca_resolve(int M, long long N, const dpca_params *in, dpca_params *out)
+{
+    if (!in || !out) {
+        set_error("params is NULL");
+        return -1;
+    }
+    dpca_ctx c;  // sizes and parameters only: no device state is touched
+    c.M = M;
+    c.N = N;
+    const int r = resolve_params(&c, in);
+    if (r) return r;
+    *out = c.p;
     return 0;
 }
 

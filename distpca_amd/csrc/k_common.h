@@ -5,6 +5,19 @@
 
 namespace dpca {
 
+// Elementwise kernels take a grid-stride loop: at most kEwBlocks workgroups
+// of 256 threads, so the dispatch's work-item count stays far below the 2^32
+// an AMD dispatch packet can describe (a 25 GB matrix has 2.5e10 bytes).
+constexpr int64_t kEwBlocks = 1 << 20;
+inline unsigned ew_blocks(int64_t n)
+{
+    const int64_t b = (n + 255) / 256;
+    return (unsigned)(b < kEwBlocks ? b : kEwBlocks);
+}
+#define DPCA_EW_LOOP(idx, n)                                                          \
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < (n);    \
+         idx += (int64_t)gridDim.x * blockDim.x)
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 

@@ -20,34 +20,32 @@ namespace dpca {
 __global__ void k_repack(const uint8_t *__restrict__ raw, int64_t rows, int64_t np,
                          uint8_t *__restrict__ dst, int64_t ldp, int M)
 {
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t total = rows * ldp;
-    if (idx >= total) return;
-    int64_t r = idx / ldp, b = idx - r * ldp;
-    uint8_t v = kPadByte;
-    if (b < np) {
-        v = raw[r * np + b];
-        if (b == np - 1) {
-            int valid = M - 4 * (int)(np - 1);  // 1..4 real samples in this byte
-            if (valid < 4) {
-                uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
-                v = (uint8_t)((v & keep) | (kPadByte & ~keep));
+    const int64_t total = rows * ldp;
+    DPCA_EW_LOOP(idx, total) {
+        const int64_t r = idx / ldp, b = idx - r * ldp;
+        uint8_t v = kPadByte;
+        if (b < np) {
+            v = raw[r * np + b];
+            if (b == np - 1) {
+                const int valid = M - 4 * (int)(np - 1);  // 1..4 real samples in this byte
+                if (valid < 4) {
+                    const uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
+                    v = (uint8_t)((v & keep) | (kPadByte & ~keep));
+                }
             }
         }
+        dst[idx] = v;
     }
-    dst[idx] = v;
 }
 
 __global__ void k_fill_u8(uint8_t *p, int64_t n, uint8_t v)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    DPCA_EW_LOOP(i, n) p[i] = v;
 }
 
 __global__ void k_fill_f64(double *p, int64_t n, double v)
 {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    DPCA_EW_LOOP(i, n) p[i] = v;
 }
 
 hipError_t launch_repack(hipStream_t st, const uint8_t *raw, int64_t rows, int64_t np,
@@ -55,21 +53,21 @@ hipError_t launch_repack(hipStream_t st, const uint8_t *raw, int64_t rows, int64
 {
     int64_t total = rows * ldp;
     if (total == 0) return hipSuccess;
-    k_repack<<<(unsigned)((total + 255) / 256), 256, 0, st>>>(raw, rows, np, dst, ldp, M);
+    k_repack<<<ew_blocks(total), 256, 0, st>>>(raw, rows, np, dst, ldp, M);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_u8(hipStream_t st, uint8_t *p, int64_t n, uint8_t v)
 {
     if (n == 0) return hipSuccess;
-    k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, v);
+    k_fill_u8<<<ew_blocks(n), 256, 0, st>>>(p, n, v);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v)
 {
     if (n == 0) return hipSuccess;
-    k_fill_f64<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(p, n, v);
+    k_fill_f64<<<ew_blocks(n), 256, 0, st>>>(p, n, v);
     return hipGetLastError();
 }
 
@@ -77,32 +75,30 @@ hipError_t launch_fill_f64(hipStream_t st, double *p, int64_t n, double v)
 __global__ void k_pad_rhs(const double *__restrict__ src, int M, int k, int64_t Mpad, int kp,
                           double *__restrict__ dst)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= Mpad * kp) return;
-    const int64_t i = e / kp;
-    const int j = (int)(e % kp);
-    dst[e] = (i < M && j < k) ? src[i * k + j] : 0.0;
+    DPCA_EW_LOOP(e, Mpad * kp) {
+        const int64_t i = e / kp;
+        const int j = (int)(e % kp);
+        dst[e] = (i < M && j < k) ? src[i * k + j] : 0.0;
+    }
 }
 
 __global__ void k_compact_rhs(const double *__restrict__ src, int M, int k, int kp,
                               double *__restrict__ dst)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (int64_t)M * k) return;
-    dst[e] = src[(e / k) * kp + e % k];
+    DPCA_EW_LOOP(e, (int64_t)M * k) dst[e] = src[(e / k) * kp + e % k];
 }
 
 hipError_t launch_pad_rhs(hipStream_t st, const double *src, int M, int k, int64_t Mpad, int kp,
                           double *dst)
 {
-    k_pad_rhs<<<(unsigned)((Mpad * kp + 255) / 256), 256, 0, st>>>(src, M, k, Mpad, kp, dst);
+    k_pad_rhs<<<ew_blocks(Mpad * kp), 256, 0, st>>>(src, M, k, Mpad, kp, dst);
     return hipGetLastError();
 }
 
 hipError_t launch_compact_rhs(hipStream_t st, const double *src, int M, int k, int kp,
                               double *dst)
 {
-    k_compact_rhs<<<(unsigned)(((int64_t)M * k + 255) / 256), 256, 0, st>>>(src, M, k, kp, dst);
+    k_compact_rhs<<<ew_blocks((int64_t)M * k), 256, 0, st>>>(src, M, k, kp, dst);
     return hipGetLastError();
 }
 
@@ -112,18 +108,18 @@ hipError_t launch_compact_rhs(hipStream_t st, const double *src, int M, int k, i
 __global__ void k_scatter_cols(const uint8_t *__restrict__ src, int64_t rows, int64_t nb,
                                int n_valid, uint8_t *__restrict__ dst, int64_t ldb, int64_t off)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * nb) return;
-    const int64_t r = idx / nb, b = idx - r * nb;
-    uint8_t v = src[idx];
-    if (b == nb - 1) {
-        const int valid = n_valid - 4 * (int)(nb - 1);
-        if (valid < 4) {
-            const uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
-            v = (uint8_t)((v & keep) | (kPadByte & ~keep));
+    DPCA_EW_LOOP(idx, rows * nb) {
+        const int64_t r = idx / nb, b = idx - r * nb;
+        uint8_t v = src[idx];
+        if (b == nb - 1) {
+            const int valid = n_valid - 4 * (int)(nb - 1);
+            if (valid < 4) {
+                const uint8_t keep = (uint8_t)((1u << (2 * valid)) - 1u);
+                v = (uint8_t)((v & keep) | (kPadByte & ~keep));
+            }
         }
+        dst[r * ldb + off + b] = v;
     }
-    dst[r * ldb + off + b] = v;
 }
 
 // Fill byte columns [c0, c1) of every row with 0x55 (missing codes).
@@ -131,10 +127,10 @@ __global__ void k_fill_cols(uint8_t *__restrict__ dst, int64_t rows, int64_t ldb
                             int64_t c1)
 {
     const int64_t w = c1 - c0;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * w) return;
-    const int64_t r = idx / w;
-    dst[r * ldb + c0 + (idx - r * w)] = kPadByte;
+    DPCA_EW_LOOP(idx, rows * w) {
+        const int64_t r = idx / w;
+        dst[r * ldb + c0 + (idx - r * w)] = kPadByte;
+    }
 }
 
 hipError_t launch_scatter_cols(hipStream_t st, const uint8_t *src, int64_t rows, int64_t nb,
@@ -142,7 +138,7 @@ hipError_t launch_scatter_cols(hipStream_t st, const uint8_t *src, int64_t rows,
 {
     const int64_t n = rows * nb;
     if (n == 0) return hipSuccess;
-    k_scatter_cols<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(src, rows, nb, n_valid, dst, ldb,
+    k_scatter_cols<<<ew_blocks(n), 256, 0, st>>>(src, rows, nb, n_valid, dst, ldb,
                                                                 off);
     return hipGetLastError();
 }
@@ -152,7 +148,7 @@ hipError_t launch_fill_cols(hipStream_t st, uint8_t *dst, int64_t rows, int64_t 
 {
     const int64_t n = rows * (c1 - c0);
     if (n <= 0) return hipSuccess;
-    k_fill_cols<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(dst, rows, ldb, c0, c1);
+    k_fill_cols<<<ew_blocks(n), 256, 0, st>>>(dst, rows, ldb, c0, c1);
     return hipGetLastError();
 }
 
@@ -160,17 +156,17 @@ hipError_t launch_fill_cols(hipStream_t st, uint8_t *dst, int64_t rows, int64_t 
 __global__ void k_loadings(const double *__restrict__ T, int64_t rows, int kp, int nsv,
                            const double *__restrict__ inv_s2, double *__restrict__ W)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= rows * kp) return;
-    const int j = (int)(e % kp);
-    W[e] = (j < nsv) ? T[e] * inv_s2[j] : 0.0;
+    DPCA_EW_LOOP(e, rows * kp) {
+        const int j = (int)(e % kp);
+        W[e] = (j < nsv) ? T[e] * inv_s2[j] : 0.0;
+    }
 }
 
 hipError_t launch_loadings(hipStream_t st, const double *T, int64_t rows, int kp, int nsv,
                            const double *inv_s2, double *W)
 {
     const int64_t n = rows * kp;
-    k_loadings<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, nsv, inv_s2, W);
+    k_loadings<<<ew_blocks(n), 256, 0, st>>>(T, rows, kp, nsv, inv_s2, W);
     return hipGetLastError();
 }
 
@@ -235,6 +231,8 @@ hipError_t launch_snp_stats(hipStream_t st, const uint8_t *packed, int64_t rows,
                             double *mu, double *dsc)
 {
     if (rows == 0) return hipSuccess;
+    // one wave per row: 64 rows' work-items fit a dispatch up to 67M rows
+    if ((rows + 3) / 4 * 256 > (int64_t)UINT32_MAX) return hipErrorInvalidValue;
     k_snp_stats<<<(unsigned)((rows + 3) / 4), 256, 0, st>>>(packed, rows, ldp, inv_sqrtN, sum,
                                                             ctr, zval, mu, dsc);
     return hipGetLastError();
@@ -245,16 +243,16 @@ hipError_t launch_snp_stats(hipStream_t st, const uint8_t *packed, int64_t rows,
 __global__ void k_decode_codes(const uint8_t *__restrict__ packed, int64_t rows, int64_t ldp,
                                int M, uint8_t *__restrict__ codes)
 {
-    int64_t nb = (M + 3) / 4;
-    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * nb) return;
-    int64_t s = idx / nb, b = idx - s * nb;
-    uint8_t v = packed[s * ldp + b];
+    const int64_t nb = (M + 3) / 4;
     const uint8_t map[4] = {0, 3, 1, 2};
+    DPCA_EW_LOOP(idx, rows * nb) {
+        const int64_t s = idx / nb, b = idx - s * nb;
+        const uint8_t v = packed[s * ldp + b];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        int64_t i = 4 * b + j;
-        if (i < M) codes[s * M + i] = map[(v >> (2 * j)) & 3];
+        for (int j = 0; j < 4; j++) {
+            const int64_t i = 4 * b + j;
+            if (i < M) codes[s * M + i] = map[(v >> (2 * j)) & 3];
+        }
     }
 }
 
@@ -263,7 +261,7 @@ hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t ro
 {
     int64_t n = rows * ((M + 3) / 4);
     if (n == 0) return hipSuccess;
-    k_decode_codes<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(packed, rows, ldp, M, codes);
+    k_decode_codes<<<ew_blocks(n), 256, 0, st>>>(packed, rows, ldp, M, codes);
     return hipGetLastError();
 }
 
@@ -273,12 +271,12 @@ hipError_t launch_decode_codes(hipStream_t st, const uint8_t *packed, int64_t ro
 __global__ void k_decode_f64(const uint8_t *__restrict__ packed, int64_t rows, int64_t ldp, int M,
                              const double *__restrict__ zval, double *__restrict__ out)
 {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= rows * M) return;
-    const int64_t s = e / M;
-    const int i = (int)(e % M);
-    const int code = (packed[s * ldp + (i >> 2)] >> (2 * (i & 3))) & 3;
-    out[e] = zval[4 * s + code];
+    DPCA_EW_LOOP(e, rows * M) {
+        const int64_t s = e / M;
+        const int i = (int)(e % M);
+        const int code = (packed[s * ldp + (i >> 2)] >> (2 * (i & 3))) & 3;
+        out[e] = zval[4 * s + code];
+    }
 }
 
 hipError_t launch_decode_f64(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
@@ -286,7 +284,7 @@ hipError_t launch_decode_f64(hipStream_t st, const uint8_t *packed, int64_t rows
 {
     const int64_t n = rows * M;
     if (n == 0) return hipSuccess;
-    k_decode_f64<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(packed, rows, ldp, M, zval, out);
+    k_decode_f64<<<ew_blocks(n), 256, 0, st>>>(packed, rows, ldp, M, zval, out);
     return hipGetLastError();
 }
 

@@ -10,6 +10,7 @@
 
 #include "dpca_gen.h"
 #include "dpca_internal.h"
+#include "k_common.h"
 
 namespace dpca {
 
@@ -166,25 +167,25 @@ void gen_rows_host(const GenSpec &g, const GenTables &tab, int64_t snp0, int64_t
     });
 }
 
-// One thread per packed byte of the padded device layout; bytes past the
-// real samples hold missing codes.
+// One thread per packed byte of the padded device layout (grid-stride);
+// bytes past the real samples hold missing codes.
 __global__ void k_generate(GenSpec g, const uint32_t *__restrict__ thr,
                            const uint16_t *__restrict__ wts, int64_t snp0, int64_t rows,
                            uint8_t *__restrict__ dst, int64_t ldp)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows * ldp) return;
-    const int64_t r = idx / ldp, byte = idx - r * ldp;
-    const int64_t s = snp0 + r;
-    uint32_t v = 0;
-    for (int j = 0; j < 4; j++) {
-        const int i = (int)(4 * byte + j);
-        const uint32_t c = (i < g.M && s < g.N)
-                               ? gen_code(g, s, i, gen_threshold(g, thr + s * g.npop, wts, i))
-                               : 1u;
-        v |= c << (2 * j);
+    DPCA_EW_LOOP(idx, rows * ldp) {
+        const int64_t r = idx / ldp, byte = idx - r * ldp;
+        const int64_t s = snp0 + r;
+        uint32_t v = 0;
+        for (int j = 0; j < 4; j++) {
+            const int i = (int)(4 * byte + j);
+            const uint32_t c = (i < g.M && s < g.N)
+                                   ? gen_code(g, s, i, gen_threshold(g, thr + s * g.npop, wts, i))
+                                   : 1u;
+            v |= c << (2 * j);
+        }
+        dst[idx] = (uint8_t)v;
     }
-    dst[idx] = (uint8_t)v;
 }
 
 hipError_t launch_generate(hipStream_t st, const GenSpec &g, const uint32_t *thr,
@@ -193,7 +194,7 @@ hipError_t launch_generate(hipStream_t st, const GenSpec &g, const uint32_t *thr
 {
     const int64_t n = rows * ldp;
     if (n == 0) return hipSuccess;
-    k_generate<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(g, thr, wts, snp0, rows, dst, ldp);
+    k_generate<<<ew_blocks(n), 256, 0, st>>>(g, thr, wts, snp0, rows, dst, ldp);
     return hipGetLastError();
 }
 

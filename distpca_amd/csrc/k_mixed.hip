@@ -243,29 +243,29 @@ __global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F,
                       const double *__restrict__ mu, const double *__restrict__ dsc, int64_t rps,
                       const unsigned *__restrict__ smax, u32x4 *__restrict__ UW8)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= rows / 8 * VT * 16) return;
-    const int64_t G = idx / (VT * 16);
-    const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
-    int j, c;
-    vcol(v, r, F, R, j, c);
-    uint32_t w[4] = {0, 0, 0, 0};
-    if (c >= 0) {
-        const int64_t split = (G * 8) / rps;
-        const double inv = 1.0 / slice_scale(__uint_as_float(smax[split * kp + c]));
+    DPCA_EW_LOOP(idx, rows / 8 * VT * 16) {
+        const int64_t G = idx / (VT * 16);
+        const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
+        int j, c;
+        vcol(v, r, F, R, j, c);
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (c >= 0) {
+            const int64_t split = (G * 8) / rps;
+            const double inv = 1.0 / slice_scale(__uint_as_float(smax[split * kp + c]));
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int64_t row = G * 8 + e;
-            const double t = T[row * kp + c];
-            int qu[3], qw[3];
-            slices3(dsc[row] * t, inv, qu);
-            slices3(-(dsc[row] * mu[row]) * t, inv, qw);
-            const int sh = 8 * (e & 3);
-            w[e >> 2] |= (uint32_t)(qu[j] & 0xFF) << sh;
-            w[2 + (e >> 2)] |= (uint32_t)(qw[j] & 0xFF) << sh;
+            for (int e = 0; e < 8; e++) {
+                const int64_t row = G * 8 + e;
+                const double t = T[row * kp + c];
+                int qu[3], qw[3];
+                slices3(dsc[row] * t, inv, qu);
+                slices3(-(dsc[row] * mu[row]) * t, inv, qw);
+                const int sh = 8 * (e & 3);
+                w[e >> 2] |= (uint32_t)(qu[j] & 0xFF) << sh;
+                w[2 + (e >> 2)] |= (uint32_t)(qw[j] & 0xFF) << sh;
+            }
         }
+        UW8[idx] = u32x4{w[0], w[1], w[2], w[3]};
     }
-    UW8[idx] = u32x4{w[0], w[1], w[2], w[3]};
 }
 
 hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int kp,
@@ -284,7 +284,7 @@ hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int 
         k_split_max<<<g1, 4 * kp, 0, st>>>(T, rows, kp, mu, dsc, rps, smax);
     }
     const int64_t n = rows / 8 * ct.vt() * 16;
-    k_uw8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc,
+    k_uw8<<<ew_blocks(n), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc,
                                                         rps, smax, reinterpret_cast<u32x4 *>(UW8));
     return hipGetLastError();
 }
@@ -456,24 +456,24 @@ __global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int F, 
                      int64_t seg_samples, const unsigned *__restrict__ qmax,
                      u32x4 *__restrict__ Q8)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= Mpad / 16 * VT * 16) return;
-    const int64_t G = idx / (VT * 16);
-    const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
-    int j, c;
-    vcol(v, r, F, R, j, c);
-    uint32_t w[4] = {0, 0, 0, 0};
-    if (c >= 0) {
-        const int64_t seg = (G * 16) / seg_samples;
-        const double inv = 1.0 / slice_scale(__uint_as_float(qmax[seg * kp + c]));
+    DPCA_EW_LOOP(idx, Mpad / 16 * VT * 16) {
+        const int64_t G = idx / (VT * 16);
+        const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
+        int j, c;
+        vcol(v, r, F, R, j, c);
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (c >= 0) {
+            const int64_t seg = (G * 16) / seg_samples;
+            const double inv = 1.0 / slice_scale(__uint_as_float(qmax[seg * kp + c]));
 #pragma unroll
-        for (int e = 0; e < 16; e++) {
-            int q[3];
-            slices3(Q[(G * 16 + 4 * (e & 3) + (e >> 2)) * kp + c], inv, q);
-            w[e >> 2] |= (uint32_t)(q[j] & 0xFF) << (8 * (e & 3));
+            for (int e = 0; e < 16; e++) {
+                int q[3];
+                slices3(Q[(G * 16 + 4 * (e & 3) + (e >> 2)) * kp + c], inv, q);
+                w[e >> 2] |= (uint32_t)(q[j] & 0xFF) << (8 * (e & 3));
+            }
         }
+        Q8[idx] = u32x4{w[0], w[1], w[2], w[3]};
     }
-    Q8[idx] = u32x4{w[0], w[1], w[2], w[3]};
 }
 
 hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int kp,
@@ -488,7 +488,7 @@ hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int k
     dim3 g1((unsigned)((segs + 127) / 128), (unsigned)nseg);
     k_split_max<<<g1, 4 * kp, 0, st>>>(Q, Mpad, kp, nullptr, nullptr, segs, qmax);
     const int64_t n = Mpad / 16 * ct.vt() * 16;
-    k_q8<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(Q, Mpad, kp, ct.F, ct.R, ct.vt(), segs, qmax,
+    k_q8<<<ew_blocks(n), 256, 0, st>>>(Q, Mpad, kp, ct.F, ct.R, ct.vt(), segs, qmax,
                                                        reinterpret_cast<u32x4 *>(Q8));
     return hipGetLastError();
 }

@@ -116,6 +116,13 @@ int dpca_open_synthetic(const dpca_gen_spec *g, const dpca_params *p, dpca_ctx *
 /* Writes <prefix>.bed/.bim/.fam with the same generator on the host. */
 int dpca_write_synthetic(const char *prefix, const dpca_gen_spec *g);
 
+/* The bsize and NSV/NRHS resolution of src/distpca.cpp:239-352 for one rank
+ * (auto block size from -memory, or from the host's MemTotal / 10 when
+ * memory is 0; then the clamps), without opening anything or touching a
+ * device: *out is *in with bsize, nsv, nrhs and power resolved as dpca_open
+ * resolves them.  Same error returns as dpca_open's parameter checks. */
+int dpca_resolve(int M, long long N, const dpca_params *in, dpca_params *out);
+
 /* Resolved sizes after the clamps. */
 int dpca_get_dims(dpca_ctx *c, int *M, long long *N, int *nsv, int *nrhs, int *bsize);
 

@@ -23,7 +23,15 @@ How each function is pinned (tests/test_oracle.py):
   standardize / snp_stats    closed form recomputed in numpy from genotype
                              counts; monomorphic and all-missing SNPs -> 0.
   decode_block               first-touch and cached passes agree bit for bit.
-  pass_dense                 brute-force dense Z^T (Z Q) on small inputs.
+  OOCPass                    brute-force dense Z^T (Z Q) on small inputs.
+  resolve                    against the library's own resolution
+                             (dpca_resolve) over the auto-bsize branches
+                             (fixed -memory: positive and non-positive
+                             available memory, block >= N) and the clamps.
+  write_outputs              byte-identical to the head of the reference-held
+                             docs/results/accuracy/1000_genomes_
+                             leftSingularVectors.txt rewritten from its own
+                             values and IIDs.
   orthonormalise             Q^T Q = I, Q R = A, matches numpy's QR up to signs.
   rayleigh_ritz              eigenvalues match numpy.linalg.eigh.
   rsi (whole loop)           HGDP example, nsv 20, nrhs 40, crit 2, tol 1e-3,
@@ -257,11 +265,6 @@ def rayleigh_ritz(Q: np.ndarray, AQ: np.ndarray):
     w, v, info = lapack.dsyev(B2, compute_v=1, lower=0)
     assert info == 0
     return w, v[:, ::-1].copy()
-
-
-def pass_dense(Z: np.ndarray, Q: np.ndarray) -> np.ndarray:
-    """A A^T Q = Z^T (Z Q) for a dense standardised Z (N x M)."""
-    return Z.T @ (Z @ Q)
 
 
 # ------------------------------------------------------------------ driver --

@@ -31,7 +31,8 @@ def test_params_defaults():
     assert p.prefix == b"noprefix"
 
 
-@pytest.mark.parametrize("M,k", [(5, 3), (942, 40), (1024, 20), (50000, 20), (100000, 40)])
+@pytest.mark.parametrize("M,k", [(5, 3), (942, 40), (1024, 20), (50000, 20), (100000, 40),
+                                 (400000, 40)])
 def test_library_omega_is_reference_exact(M, k):
     a = D.omega(M, k)
     b = O.omega(M, k)

@@ -1,0 +1,164 @@
+"""GPU tests at the two large shapes: 100,000 samples x 1,000,000 SNPs held in
+HBM (8 Balding-Nichols populations, Fst 0.005, 20% Dirichlet-admixed samples,
+2% missing, seed 2; nsv 20, nrhs 40, power 6) and 400,000 samples streamed
+from pageable host memory through a 2 GB device cap (10 populations, Fst
+0.005, 1% missing, seed 3; nsv 20, nrhs 40, power 4, bsize 65536).
+
+* The full 100k x 1M PCA in both forms: the mixed form stops at the FP64
+  form's iteration with every eigenvalue within 1e-5 (relative) and the 7
+  population PCs within 1e-4 rad.
+* A 16,384-SNP slice of the same generator at M = 100,000 -- past M = 46,341,
+  where the reference's Omega seed jj*M + kk wraps (src/distpca.cpp:389-394)
+  -- against the oracle: per-SNP statistics bit-exact, the FP64 pass within
+  1e-12 and the mixed pass within 2e-5 of the pass norm, and a two-iteration
+  FP64 run seeded with the library's own Omega against the oracle's run from
+  the oracle's Omega (eigenvalues 1e-10, angles 1e-8 rad; the oracle's own
+  block-size spread there is 3e-15 / 2e-12).
+* 400,000 x 250,000 (25 GB) streamed from pageable memory with a 2 GB cap
+  against the same matrix resident in HBM, both forms; a 4,096-SNP slice at
+  M = 400,000 streamed the same way against the oracle.
+"""
+import numpy as np
+import pytest
+
+import distpca_amd as D
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+C3 = dict(npop=8, fst=0.005, admix_frac=0.2, missing=0.02, seed=2)
+C4 = dict(npop=10, fst=0.005, missing=0.01, seed=3)
+
+
+def _banner(msg):
+    print(f"[gpu] {msg}", flush=True)
+
+
+def _angles(U, V, js):
+    return {j: float(O.principal_angles(U[:, :j], V[:, :j]).max()) for j in js}
+
+
+def test_c3_full_mixed_vs_fp64():
+    M, N = 100000, 1000000
+    spec = D.gen_spec(M, N, **C3)
+    base = dict(nsv=20, nrhs=40, power=6)
+    out = {}
+    for name, prec in (("mixed", D.MIXED), ("fp64", D.FP64)):
+        _banner(f"c3 full PCA {M} x {N}, {name}")
+        with D.PCA.synthetic(spec, D.Params(precision=prec, **base)) as pca:
+            out[name] = pca.run()
+    (sm, Um, rm), (sf, Uf, rf) = out["mixed"], out["fp64"]
+    rel = np.abs(sm ** 2 - sf ** 2) / sf ** 2
+    ang = _angles(Um, Uf, (1, 4, 7, 20))
+    _banner(f"c3: its mixed {rm['total_its']} fp64 {rf['total_its']}; eigenvalue rel err "
+            f"max {rel.max():.2e}; angles {ang}")
+    assert rm["total_its"] == rf["total_its"]
+    assert rm["passes"] == rf["passes"] == 7 * rf["total_its"]
+    assert rel.max() <= 1e-5
+    assert ang[7] <= 1e-4
+
+
+@pytest.fixture(scope="module")
+def c3_slice(tmp_path_factory):
+    """The first 16,384 SNPs of the 100k x 1M generator as a PLINK triple
+    (the generator is counter-based per SNP, so a shorter N gives the same
+    rows)."""
+    M, N = 100000, 16384
+    pre = str(tmp_path_factory.mktemp("c3s") / "c3s")
+    D.write_synthetic(pre, D.gen_spec(M, N, **C3))
+    M2, N2, _, _, packed = O.read_plink(pre)
+    assert (M2, N2) == (M, N)
+    return pre, M, N, packed
+
+
+def test_c3_slice_stats_and_passes(c3_slice):
+    pre, M, N, packed = c3_slice
+    _banner(f"c3 slice {M} x {N}: stats and one pass in both forms vs the oracle")
+    s0, c0, _, _, z0 = O.snp_stats(packed, N, M)
+    Q = np.linalg.qr(D.omega(M, 40))[0]
+    ref = O.OOCPass(packed, M, N, 4096)(Q)
+    for prec, tol in ((D.FP64, 1e-12), (D.MIXED, 2e-5)):
+        with D.PCA.from_bfile(pre, D.Params(nsv=20, nrhs=40, power=6, bsize=4096,
+                                            precision=prec)) as pca:
+            s, c, z = pca.snp_stats()
+            got = pca.pass_(Q)
+        assert np.array_equal(s, s0) and np.array_equal(c, c0)
+        assert np.array_equal(z.view(np.int64), z0.view(np.int64))
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        _banner(f"c3 slice pass precision {prec}: rel err {err:.2e}")
+        assert err < tol, (prec, err)
+
+
+def test_c3_slice_run_library_omega(c3_slice):
+    pre, M, N, packed = c3_slice
+    _banner(f"c3 slice {M} x {N}: two-iteration FP64 run, library Omega vs oracle Omega")
+    kw = dict(nsv=20, nrhs=40, power=1, crit=1, tol=1e-3, miter=2, bsize=4096)
+    r = O.rsi(packed, M, N, O.resolve(O.Params(**kw), M, N))      # O.omega inside
+    with D.PCA.from_bfile(pre, D.Params(**kw)) as pca:
+        sigma, rhs, rep = pca.run()                               # D.omega (library)
+    rel = np.abs(sigma ** 2 - r.sigma ** 2) / r.sigma ** 2
+    ang = _angles(rhs, r.rhs, (1, 2, 4, 7, 10, 20))
+    _banner(f"c3 slice run: eigenvalue rel err max {rel.max():.2e}; angles {ang}")
+    assert rep["total_its"] == r.total_its == 2 and rep["passes"] == r.passes == 4
+    assert rel.max() <= 1e-10
+    assert max(ang.values()) <= 1e-8
+
+
+def test_c4_shape_streamed_pageable_vs_resident():
+    M, N = 400000, 250000
+    spec = D.gen_spec(M, N, **C4)
+    Q = np.linalg.qr(D.omega(M, 40))[0]
+    for prec in (D.FP64, D.MIXED):
+        base = dict(nsv=20, nrhs=40, power=4, crit=1, tol=1e-3, miter=2, bsize=65536,
+                    precision=prec)
+        _banner(f"c4 shape {M} x {N}: resident vs pageable streamed (2 GB cap), precision {prec}")
+        with D.PCA.synthetic(spec, D.Params(**base)) as res, \
+                D.PCA.synthetic(spec, D.Params(device_cap=int(2e9), host_mode=D.HOST_PAGEABLE,
+                                               **base)) as stm:
+            s1, c1, z1 = res.snp_stats()
+            s2, c2, z2 = stm.snp_stats()
+            assert np.array_equal(s1, s2) and np.array_equal(c1, c2)
+            assert np.array_equal(z1.view(np.int64), z2.view(np.int64))
+            assert np.array_equal(res.packed(N - 1000, 1000), stm.packed(N - 1000, 1000))
+            a = res.pass_(Q)
+            b1 = stm.pass_(Q)            # every block copied; leading blocks kept
+            b2 = stm.pass_(Q)            # resident blocks + the streamed rest
+            sa, Ua, ra = res.run()
+            sb, Ub, rb = stm.run()
+        tol = 1e-12 if prec == D.FP64 else 2e-6
+        for b in (b1, b2):
+            err = np.abs(a - b).max() / np.abs(a).max()
+            assert err < tol, (prec, err)
+        assert np.array_equal(b1, b2)
+        assert rb["streamed"] == 1 and rb["host_mode"] == D.HOST_PAGEABLE
+        assert 0 < rb["resident_bytes"] < 2e9
+        assert ra["total_its"] == rb["total_its"] == 2
+        rel = np.abs(sa ** 2 - sb ** 2) / sa ** 2
+        ang = _angles(Ua, Ub, (1, 9, 20))
+        _banner(f"c4 shape precision {prec}: eig rel diff {rel.max():.2e}, angles {ang}")
+        assert rel.max() <= (1e-10 if prec == D.FP64 else 1e-6)
+        assert ang[9] <= (1e-8 if prec == D.FP64 else 1e-4)
+
+
+def test_c4_slice_streamed_vs_oracle():
+    M, N = 400000, 4096
+    _banner(f"c4 slice {M} x {N} streamed from pageable memory vs the oracle")
+    with D.PCA.synthetic(D.gen_spec(M, N, **C4), D.Params(nsv=2, nrhs=4)) as g:
+        packed = g.packed()
+    s0, c0, _, _, z0 = O.snp_stats(packed, N, M)
+    Q = np.linalg.qr(D.omega(M, 40))[0]
+    ref = O.OOCPass(packed, M, N, 1024)(Q)
+    for prec, tol in ((D.FP64, 1e-12), (D.MIXED, 2e-5)):
+        # 256 MB cap: 1,024-row (100 MB) blocks, none of them resident
+        p = D.Params(nsv=20, nrhs=40, power=4, miter=1, bsize=1024, precision=prec,
+                     device_cap=256 << 20, host_mode=D.HOST_PAGEABLE)
+        with D.PCA.from_packed(packed, M, N, p) as pca:
+            got = pca.pass_(Q)
+            s, c, z = pca.snp_stats()
+            _, _, rep = pca.run(Q)  # a short run only to read the streaming report
+        assert np.array_equal(s, s0) and np.array_equal(c, c0)
+        assert np.array_equal(z.view(np.int64), z0.view(np.int64))
+        assert rep["streamed"] == 1 and rep["resident_bytes"] == 0
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        _banner(f"c4 slice pass precision {prec}: rel err {err:.2e}")
+        assert err < tol, (prec, err)

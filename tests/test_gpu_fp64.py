@@ -114,19 +114,28 @@ def test_pass_column_groups(k):
 
 
 @pytest.mark.parametrize("k", [40, 64])
-def test_pass_wide_tiles_streamed(k):
+def test_pass_wide_tiles_streamed(k, capfd, monkeypatch):
     # two or more whole column tiles take the 4-SNP-tile FP64 Z Q where it
     # keeps two workgroups per CU; streamed 1024-row blocks also split the
-    # samples into segments (short blocks): both against the oracle
+    # samples into segments (short blocks): both against the oracle.  The
+    # launch plan line confirms that the 4-tile form is the one that ran at
+    # nrhs 40; at 64 (four whole tiles) its accumulators leave one workgroup
+    # per CU, so the 2-tile form runs there.
     M, N = 2000, 6000
     _banner(f"wide-tile pass k={k}, resident and streamed")
     packed = _packed_random(M, N, seed=300 + k)
     Q = np.random.default_rng(300 + k).standard_normal((M, k))
     ref = O.OOCPass(packed, M, N, 512)(Q)
     base = dict(nsv=2, nrhs=k, bsize=512)
+    monkeypatch.setenv("DPCA_SHOW_PLAN", "1")
+    capfd.readouterr()
     with D.PCA.from_packed(packed, M, N, D.Params(**base)) as res, \
             D.PCA.from_packed(packed, M, N, D.Params(device_cap=1 << 20, **base)) as stm:
         a, b = res.pass_(Q), stm.pass_(Q)
+    plans = [l for l in capfd.readouterr().err.splitlines() if l.startswith("dpca plan:")]
+    assert len(plans) == 2, plans
+    for l in plans:
+        assert "fp64 splits" in l and f"Z Q tiles {4 if k == 40 else 2};" in l, l
     for got in (a, b):
         err = np.abs(got - ref).max() / np.abs(ref).max()
         assert err < 1e-12, err
@@ -314,6 +323,41 @@ def test_cli_dropin(tmp_path):
         a = O.principal_angles(U[:, :j], r.rhs[:, :j]).max()
         sp = O.principal_angles(ra.rhs[:, :j], r.rhs[:, :j]).max()
         assert a <= max(1e-8, 20 * sp), (j, a, sp)
+    # byte layout: the oracle's writer (pinned on the reference-held 1000
+    # Genomes output) rewrites the parsed values to the same bytes
+    O.write_outputs(str(tmp_path / "orc"), sv, U, 4, iids)
+    for f in ("_singularValues.txt", "_leftSingularVectors.txt"):
+        assert open(str(tmp_path / ("res" + f))).read() == open(str(tmp_path / ("orc" + f))).read()
+
+
+def _fetch_line(stdout):
+    key = "Total number of times which the data matrix was fetched from the memory: "
+    return int(next(l for l in stdout.splitlines() if l.startswith(key))[len(key):])
+
+
+@pytest.mark.parametrize("strict", [1, 0])
+def test_cli_fetch_count(tmp_path, strict):
+    # the console's fetch count: total_its * (power + 1) under the
+    # reference's schedule (src/utilities.cpp:160-161); the passes actually
+    # made under -strict 0 (power + 1 for the first iteration, power after)
+    _banner(f"CLI fetch count, strict {strict}")
+    pre = str(tmp_path / "fc")
+    D.write_synthetic(pre, D.gen_spec(600, 4000, npop=3, fst=0.05, missing=0.01, seed=5))
+    kw = dict(nsv=4, nrhs=8, power=3, crit=1, tol=1e-6, bsize=500, strict_passes=strict)
+    out = subprocess.run([D.EXE_PATH, "-bfile", pre, "-nsv", "4", "-nrhs", "8", "-power", "3",
+                          "-crit", "1", "-tol", "1e-6", "-bsize", "500", "-strict", str(strict)],
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    with D.PCA.from_bfile(pre, D.Params(**kw)) as pca:
+        assert (pca.M, pca.N, pca.nsv, pca.nrhs, pca.bsize) == \
+            (600, 4000) + tuple(D.resolve(600, 4000, D.Params(**kw))[f]
+                                for f in ("nsv", "nrhs", "bsize"))
+        _, _, rep = pca.run()
+    its = rep["total_its"]
+    assert its > 1
+    want = its * 4 if strict else 4 + 3 * (its - 1)
+    assert rep["passes"] == want
+    assert _fetch_line(out.stdout) == want
 
 
 @pytest.mark.parametrize("incore", [True, False])
@@ -442,3 +486,20 @@ def test_streamed_host_modes(tmp_path, mode):
     assert ra["total_its"] == rb["total_its"]
     assert np.allclose(sa, sb, rtol=1e-10)
     assert O.principal_angles(Ua[:, :2], Ub[:, :2]).max() < 1e-8
+    # against the oracle itself (the reference's MPI-IO block loop,
+    # src/methods.cpp:795-811, restated): per-SNP statistics bit-exact, the
+    # streamed pass within 1e-12, and the streamed run's iteration count,
+    # eigenvalues and the 2 population PCs within the FP64 tolerances (the
+    # third PC lies in the noise bulk and is compared with the resident run
+    # above)
+    M2, N2, _, _, packed = O.read_plink(pre)
+    os_, oc_, _, _, oz_ = O.snp_stats(packed, N2, M2)
+    assert np.array_equal(s2, os_) and np.array_equal(c2, oc_)
+    assert np.array_equal(z2.view(np.int64), oz_.view(np.int64))
+    ref_pass = O.OOCPass(packed, M, N, 3000)(Q)
+    assert np.abs(b - ref_pass).max() / np.abs(ref_pass).max() < 1e-12
+    r = O.rsi(packed, M, N, O.resolve(O.Params(nsv=3, nrhs=6, power=2, crit=1, tol=1e-10,
+                                               miter=6, bsize=3000), M, N))
+    assert rb["total_its"] == r.total_its and rb["passes"] == r.passes
+    assert np.allclose(sb[:2] ** 2, r.sigma[:2] ** 2, rtol=1e-10, atol=0)
+    assert O.principal_angles(Ub[:, :2], r.rhs[:, :2]).max() < 1e-8

@@ -36,7 +36,8 @@ def test_park_miller_known_answer():
 
 
 @pytest.mark.skipif(not os.path.exists(REF_LIB), reason="oracle/_ref not built")
-@pytest.mark.parametrize("M,k", [(942, 40), (1024, 20), (50000, 20), (100000, 40)])
+@pytest.mark.parametrize("M,k", [(942, 40), (1024, 20), (50000, 20), (100000, 40),
+                                 (400000, 40)])
 def test_omega_matches_compiled_reference(M, k):
     a = O.omega(M, k)
     b = _ref_omega(M, k)
@@ -225,3 +226,66 @@ def test_full_svd_diag_closed_forms():
     assert abs(d["frob_norm_angle"] - 2 * np.sqrt(1 - np.cos(th))) < 1e-12
     assert abs(d["cos_error"] - 2 * np.sin(th / 2)) < 1e-12
     assert np.allclose(d["sigma_relerr"], 0.01)
+
+
+def _fixture(*parts):
+    return os.path.join(ROOT, "tests", "golden", *parts)
+
+
+def test_write_outputs_matches_reference_file_layout(tmp_path):
+    # the head of docs/results/accuracy/1000_genomes_leftSingularVectors.txt
+    # (reference-held output, 10 PCs): rewriting its own IIDs and values with
+    # the oracle's writer must give the same bytes ("%8s" IID, "\t%2.13f")
+    ref = open(_fixture("formats", "leftSingularVectors_head.txt")).read()
+    lines = ref.splitlines()
+    nsv = len(lines[0].split("\t")) - 1
+    iids = [ln.split("\t")[0].strip() for ln in lines[1:]]
+    rhs = np.array([[float(v) for v in ln.split("\t")[1:]] for ln in lines[1:]])
+    sigma = np.linspace(3.0, 1.0, nsv)
+    pre = str(tmp_path / "res")
+    O.write_outputs(pre, sigma, rhs, nsv, iids)
+    assert open(pre + "_leftSingularVectors.txt").read() == ref
+    sv = open(pre + "_singularValues.txt").read().splitlines()
+    assert sv == ["%.13f" % s for s in sigma] and sv[0] == "3.0000000000000"
+
+
+def _ram_gb():
+    for ln in open("/proc/meminfo"):
+        if ln.startswith("MemTotal:"):
+            return int(ln.split()[1]) / 1e6
+    raise RuntimeError("no MemTotal")
+
+
+@pytest.mark.parametrize("M,N,kw", [
+    (1024, 50000, dict(nsv=10, nrhs=20, bsize=4096)),             # given, < N
+    (1024, 50000, dict(nsv=10, nrhs=20, bsize=60000)),            # > N -> N (in-core)
+    (1024, 50000, dict(nsv=10, nrhs=20, bsize=-3)),               # auto
+    (100000, 1000000, dict(nsv=20, nrhs=40, memory=2.0)),         # auto from -memory
+    (100000, 1000000, dict(nsv=20, nrhs=40, memory=0.05)),        # available <= 0
+    (20000, 500000, dict(nsv=10, nrhs=20, memory=500.0)),         # auto block >= N
+    (942, 133594, dict(nsv=20, nrhs=40)),                         # RAM / 10
+    (30, 5000, dict(nsv=40, nrhs=0, bsize=100)),                  # nsv > min, nrhs <= 0
+    (30, 5000, dict(nsv=8, nrhs=50, bsize=100)),                  # nrhs > min
+    (500, 5000, dict(nsv=12, nrhs=5, bsize=100)),                 # nrhs < nsv
+    (500, 20, dict(nsv=12, nrhs=30, bsize=0, memory=1.0)),        # N < M
+])
+def test_resolve_matches_library(M, N, kw):
+    # src/distpca.cpp:239-352: the oracle's restatement and the library's
+    # dpca_resolve (host-only) agree on every branch
+    import distpca_amd as D
+    fields = dict(kw)
+    ram = _ram_gb()
+    ref = O.resolve(O.Params(**fields), M, N, ram_gb=ram)
+    got = D.resolve(M, N, D.Params(**fields))
+    assert (got["bsize"], got["nsv"], got["nrhs"]) == (ref.bsize, ref.nsv, ref.nrhs), (got, ref)
+
+
+def test_resolve_auto_bsize_closed_form():
+    # the auto branch by hand at 100k x 1M, -memory 2 (GB): (2e9 - membuff) / (8 M)
+    M, N, nsv, nrhs = 100000, 1000000, 20, 40
+    membuff = 3 * N * 8 + M * nsv * 8 + 2 * N * nrhs + 3 * M * 8 + 2048 * 100000
+    assert O.resolve(O.Params(nsv=nsv, nrhs=nrhs, memory=2.0), M, N).bsize == \
+        int((2e9 - membuff) / (8 * M)) == 2091
+    # non-positive available memory: (membuff + 8 M) / 1e7
+    assert O.resolve(O.Params(nsv=nsv, nrhs=nrhs, memory=0.05), M, N).bsize == \
+        int((membuff + 8 * M) / 1e7)
