@@ -99,7 +99,7 @@ EXPORTS = [
     "dpca_model_free", "dpca_server_params_init", "dpca_server_create",
     "dpca_server_submit_project", "dpca_server_submit_job", "dpca_server_wait",
     "dpca_server_stats", "dpca_server_destroy", "dpca_host_alloc", "dpca_host_free",
-    "dpca_full_svd", "dpca_write_fullsvd",
+    "dpca_full_svd", "dpca_write_fullsvd", "dpca_orthonormalize_noalloc",
 ]
 
 _lib = None
@@ -133,6 +133,7 @@ def lib():
     L.dpca_write_outputs.argtypes = [vp, ctypes.c_char_p, vp, vp]
     L.dpca_orthonormalize.argtypes = [vp, i64, i32, vp, P(i32), i32]
     L.dpca_syev.argtypes = [vp, i32, vp, vp, i32]
+    L.dpca_orthonormalize_noalloc.argtypes = [vp, i64, i32, vp, P(i32), i32, vp]
     L.dpca_bench_pass.argtypes = [vp, vp, i32, vp]
     L.dpca_close.argtypes = [vp]
     L.dpca_close.restype = None

@@ -174,6 +174,23 @@ int dpca_write_outputs(dpca_ctx *c, const char *prefix, const double *rhs,
 int dpca_orthonormalize(const double *A, long long m, int k, double *Q, int *shifted,
                         int device);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dpca_orthonormalize on buffers already on the device,
+ * with its intermediates (Gram partial sums, Gram matrix, R^{-1}, fallback
+ * flags) in a static 8.4 MB device array per GPU instead of allocations.
+ * A, Q: m x kp row-major device buffers, kp = k rounded up to 16, columns
+ * k..kp-1 zero; A is overwritten (it holds the second CholQR pass), Q gets
+ * the orthonormal factor.  Limits: k <= 64, m x kp < 2^31 (else -1 and a
+ * message).  stream: a hipStream_t (NULL: the null stream).  With shifted
+ * NULL the call only enqueues work (no synchronisation, so it may be
+ * captured); otherwise it synchronises the stream, stores whether the shifted
+ * fallback ran and returns -4 if even that failed.  Calls that overlap on one
+ * GPU share the array and race.  Allocates nothing: no hipMalloc, hipFree or
+ * allocating library call. */
+int dpca_orthonormalize_noalloc(double *A, long long m, int k, double *Q, int *shifted,
+                                int device, void *stream);
+
 /* Symmetric eigensolver unit (replaces LAPACKE_dsyev('V','U') plus the column
  * reversal, src/methods.cpp:980-1000): upper triangle of the k x k B;
  * eigenvalues descending in w, unit eigenvectors as the columns of X. */
