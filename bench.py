@@ -252,6 +252,8 @@ def main():
                     help="skip the full PCA runs of both forms (and their accuracy comparison)")
     ap.add_argument("--full", action="store_true", help=argparse.SUPPRESS)  # the default
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--slices", type=int, default=0,
+                    help="int8 slices of the mixed form's FP64 operand (2 or 3; 0 = default)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
     a = ap.parse_args()
@@ -302,7 +304,7 @@ def main():
         return D.PCA.synthetic(spec, params)
 
     pm = D.Params(nsv=nsv, nrhs=nrhs, power=power, bsize=bsize, precision=D.MIXED,
-                  device_cap=cap, verbose=2)
+                  device_cap=cap, verbose=2, slices=a.slices)
     mixed = open_ctx(pm)
     log(f"generated + stats in {time.perf_counter() - t0:.1f} s")
     Q = np.linalg.qr(D.omega(M, nrhs))[0]

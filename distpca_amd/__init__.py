@@ -32,6 +32,7 @@ class _Params(ctypes.Structure):
         ("prefix", ctypes.c_char * 1024), ("precision", ctypes.c_int),
         ("device", ctypes.c_int), ("device_cap", ctypes.c_longlong),
         ("strict_passes", ctypes.c_int), ("host_mode", ctypes.c_int),
+        ("slices", ctypes.c_int),
     ]
 
 
@@ -189,12 +190,14 @@ class Params:
     device_cap: int = 0
     strict_passes: int = 1
     host_mode: int = HOST_PINNED
+    slices: int = 0
 
     def c(self) -> _Params:
         p = _Params()
         lib().dpca_params_init(ctypes.byref(p))
         for f in ("nsv", "nrhs", "power", "crit", "tol", "bsize", "miter", "verbose",
-                  "fwrite", "memory", "precision", "device", "device_cap", "strict_passes", "host_mode"):
+                  "fwrite", "memory", "precision", "device", "device_cap", "strict_passes", "host_mode",
+                  "slices"):
             setattr(p, f, getattr(self, f))
         p.prefix = self.prefix.encode()
         return p

@@ -8,7 +8,8 @@
 // matrix before it streams from host memory>, -strict 0 (reuse the last pass
 // of an iteration as the next iteration's first: power passes per iteration
 // instead of power+1), -hostmem pinned|pageable|file (where a streamed matrix
-// lives between passes; file re-reads the .bed every pass).
+// lives between passes; file re-reads the .bed every pass), -slices 2|3 (int8
+// slices of the mixed form's FP64 operand).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -208,6 +209,7 @@ int main(int argc, char **argv)
     findarg("power", INT, &p.power, argc, argv);
     findarg("fullSVD", INT, &p.fullsvd, argc, argv);
     findarg("device", INT, &p.device, argc, argv);
+    findarg("slices", INT, &p.slices, argc, argv);
     int strict = 1;
     if (findarg("strict", INT, &strict, argc, argv)) p.strict_passes = strict != 0;
     double devcap_gb = 0;

@@ -179,6 +179,8 @@ struct dpca_ctx {
     unsigned *qmax = nullptr;         // per (sample segment, column) max |Q|
     float *Tpart = nullptr, *Apart = nullptr;
     int nseg = 1, st_tiles = 8, nsplit = 1, sa_tiles = 4;
+    int ns = 3;                       // int8 slices of the FP64 operand
+    int za_waves = 8, zt_waves = 8;   // waves per int8 product workgroup
     int64_t seg_bytes = 0, rps = 0;
     int nseg64 = 1;                   // FP64 Z Q sample segments
     int64_t seg64 = 0;
@@ -488,29 +490,34 @@ static int alloc_work(dpca_ctx *c)
         c->nseg = (int)((c->ldp + c->seg_bytes - 1) / c->seg_bytes);
         // serving jobs keep the two-tile forms (fewer resident waves, room
         // for the projections beside them)
-        const bool two = g_job_ctx && col_tiles(c->k).vt() <= 9;
-        c->st_tiles = two ? 2 : za_i8_tiles(c->k);
-        // tuning knob: 1 or 2 (anything else keeps the computed choice)
+        const int nsl = c->ns;
+        // tuning knobs: DPCA_I8_WAVES (8, or 4: one wave per SIMD with larger
+        // tiles), DPCA_ZA_TILES (1, 2, 4) and DPCA_ZT_TILES (2, 4, 8) where
+        // the accumulators fit (anything else keeps the computed choice)
+        if (const char *e = getenv("DPCA_I8_WAVES"))
+            if (!strcmp(e, "4")) c->za_waves = c->zt_waves = 4;
+        const bool two = g_job_ctx && za_i8_valid(c->k, nsl, 2, c->za_waves);
+        c->st_tiles = two ? 2 : za_i8_tiles(c->k, nsl, c->za_waves);
         if (const char *e = getenv("DPCA_ZA_TILES"))
-            if (!strcmp(e, "1") || !strcmp(e, "2")) c->st_tiles = atoi(e);
+            if (za_i8_valid(c->k, nsl, atoi(e), c->za_waves)) c->st_tiles = atoi(e);
         // Z^T T: SNP splits of <= 32768 rows (fewer partials), >= ~1024
         // workgroups
-        c->sa_tiles = (two && c->ldp % 128 == 0) ? 4 : zt_i8_tiles(c->k, c->ldp);
-        // tuning knob: 2 or 4, and 4 only where the row width is a multiple
-        // of its 128-byte tile (anything else keeps the computed choice)
+        c->sa_tiles = (two && zt_i8_valid(c->k, nsl, 4, c->ldp, c->zt_waves))
+                          ? 4
+                          : zt_i8_tiles(c->k, nsl, c->ldp, c->zt_waves);
         if (const char *e = getenv("DPCA_ZT_TILES"))
-            if (!strcmp(e, "2") || (!strcmp(e, "4") && c->ldp % 128 == 0)) c->sa_tiles = atoi(e);
-        const int64_t tl = c->ldp / (c->sa_tiles * 32);
+            if (zt_i8_valid(c->k, nsl, atoi(e), c->ldp, c->zt_waves)) c->sa_tiles = atoi(e);
+        const int64_t tl = c->ldp / zt_i8_tile_bytes(c->sa_tiles, c->zt_waves);
         int64_t ns = std::max<int64_t>((1024 + tl - 1) / tl, (B + 32767) / 32768);
         ns = std::min<int64_t>(ns, B / 128);
         // up to 4x more splits where they fill the last round of resident
         // workgroups better
         if (!g_job_ctx)
             ns = fit_splits(tl, B, 128, ns, ns, std::min<int64_t>(4 * ns, B / 128),
-                            device_slots(zt_i8_blocks_per_cu(c->k, c->sa_tiles)));
+                            device_slots(zt_i8_blocks_per_cu(c->k, nsl, c->sa_tiles, c->zt_waves)));
         c->rps = round_up((B + ns - 1) / ns, 128);
         c->nsplit = (int)((B + c->rps - 1) / c->rps);
-        const int vt = col_tiles(c->k).vt();
+        const int vt = col_tiles(c->k, nsl).vt();
         r |= dalloc(&c->Q8, (size_t)c->Mpad / 16 * vt * 16 * sizeof(uint4));
         r |= dalloc(&c->qmax, (size_t)c->nseg * c->kp * sizeof(unsigned));
         r |= dalloc(&c->UW8, (size_t)c->Npad / 8 * vt * 16 * sizeof(uint4));
@@ -572,6 +579,13 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
     if (r) {
         delete c;
         return r;
+    }
+    if (c->p.slices == 0) c->p.slices = 3;
+    c->ns = c->p.slices;
+    if (c->p.slices != 2 && c->p.slices != 3) {
+        set_error("slices must be 2 or 3 (0 for the default)");
+        delete c;
+        return -1;
     }
     if (c->p.precision != DPCA_FP64 && c->p.precision != DPCA_MIXED) {
         set_error("unknown precision %d", c->p.precision);
@@ -665,10 +679,11 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
     if (getenv("DPCA_SHOW_PLAN"))  // launch shapes, for tuning
         fprintf(stderr,
                 "dpca plan: block rows %lld; fp64 splits %d, Z Q segments %d, Z Q tiles %d; "
-                "int8 Z Q tiles %d segments %d, Z^T T tiles %d splits %d (%lld rows)\n",
+                "int8 slices %d, Z Q tiles %d x %d waves segments %d, Z^T T tiles %d x %d waves "
+                "splits %d (%lld rows)\n",
                 (long long)c->block_rows, c->splits, c->nseg64,
-                za_f64_rows_per_wg(c->k, c->kp) / 64, c->st_tiles, c->nseg, c->sa_tiles,
-                c->nsplit, (long long)c->rps);
+                za_f64_rows_per_wg(c->k, c->kp) / 64, c->ns, c->st_tiles, c->za_waves, c->nseg,
+                c->sa_tiles, c->zt_waves, c->nsplit, (long long)c->rps);
     if (!c->streamed && (r = dalloc(&c->packed, (size_t)bytes))) {
         free_ctx(c);
         return r;
@@ -879,7 +894,7 @@ static int pass_prologue(dpca_ctx *c, const double *Qin)
 {
     if (c->p.precision != DPCA_MIXED) return 0;
     Stage s(c, T_MM);
-    DPCA_CHECK(launch_q8(c->st, Qin, c->Mpad, c->k, c->kp, c->seg_bytes, c->qmax, c->Q8));
+    DPCA_CHECK(launch_q8(c->st, Qin, c->Mpad, c->k, c->ns, c->kp, c->seg_bytes, c->qmax, c->Q8));
     return 0;
 }
 
@@ -907,7 +922,8 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     {
         Stage s(c, T_MM_A);
         DPCA_CHECK(launch_za_i8(c->st, blk, rows, c->ldp, c->Q8, c->qmax, c->mu + r0,
-                                c->dsc + r0, c->k, c->st_tiles, c->seg_bytes, c->nseg,
+                                c->dsc + r0, c->k, c->ns, c->st_tiles, c->za_waves, c->seg_bytes,
+                                c->nseg,
                                 c->Tpart));
     }
     {
@@ -918,15 +934,16 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     }
     if (half) return 0;
     const int nsplit = (int)((rows + c->rps - 1) / c->rps);
-    uint4 *UW8 = c->UW8 + (r0 / 8) * col_tiles(c->k).vt() * 16;
+    uint4 *UW8 = c->UW8 + (r0 / 8) * col_tiles(c->k, c->ns).vt() * 16;
     {
         Stage s(c, T_MM);
-        DPCA_CHECK(launch_uw8(c->st, T, rows, c->k, c->kp, c->mu + r0, c->dsc + r0, c->rps,
+        DPCA_CHECK(launch_uw8(c->st, T, rows, c->k, c->ns, c->kp, c->mu + r0, c->dsc + r0, c->rps,
                               c->smax, UW8, true));
     }
     {
         Stage s(c, T_MM_AT);
-        DPCA_CHECK(launch_zt_i8(c->st, blk, rows, c->ldp, UW8, c->smax, c->k, c->sa_tiles,
+        DPCA_CHECK(launch_zt_i8(c->st, blk, rows, c->ldp, UW8, c->smax, c->k, c->ns, c->sa_tiles,
+                                c->zt_waves,
                                 c->rps, nsplit, c->Apart));
     }
     Stage s(c, T_MM);
@@ -1388,6 +1405,7 @@ int dpca_params_init(dpca_params *p)
     p->device_cap = 0;
     p->strict_passes = 1;
     p->host_mode = DPCA_HOST_PINNED;
+    p->slices = 0;
     return 0;
 }
 
@@ -2179,6 +2197,7 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     dpca_model *m = new dpca_model();
     m->dev = c->dev;
     m->precision = c->p.precision;
+    m->ns = c->ns;
     m->N = c->N;
     m->Npad = c->Npad;
     m->nsv = nsv;
@@ -2191,7 +2210,7 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     r |= dalloc(&m->dsc, n1);
     r |= dalloc(&m->W, n1 * c->kp);
     const int64_t prps = proj_rps_mx(c->Npad);
-    const size_t uw8_bytes = (size_t)c->Npad / 8 * col_tiles(c->k).vt() * 16 * sizeof(uint4);
+    const size_t uw8_bytes = (size_t)c->Npad / 8 * col_tiles(c->k, c->ns).vt() * 16 * sizeof(uint4);
     const size_t smax_bytes = (size_t)((c->Npad + prps - 1) / prps) * c->kp * sizeof(unsigned);
     if (m->precision == DPCA_MIXED) {
         r |= dalloc(&m->UW8, uw8_bytes);
@@ -2232,7 +2251,8 @@ int dpca_model_from_run(dpca_ctx *c, const double *rhs, const double *sigma, int
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->mu, c->mu, n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess) e = hipMemcpyAsync(m->dsc, c->dsc, n1, hipMemcpyDeviceToDevice, c->st);
     if (!r && e == hipSuccess && m->UW8)
-        e = launch_uw8(c->st, m->W, c->Npad, c->k, c->kp, m->mu, m->dsc, prps, m->smax, m->UW8, false);
+        e = launch_uw8(c->st, m->W, c->Npad, c->k, c->ns, c->kp, m->mu, m->dsc, prps, m->smax, m->UW8,
+                       false);
     if (!r && e == hipSuccess) e = hipStreamSynchronize(c->st);
     collect_timers(c);
     (void)hipFree(inv);

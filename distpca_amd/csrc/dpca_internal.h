@@ -105,7 +105,7 @@ hipError_t launch_sum_parts_f32(hipStream_t st, const float *part, int nparts, i
 hipError_t launch_reduce_parts(hipStream_t st, const double *part, int splits, int64_t n,
                                double *out, int accumulate);
 
-// Mixed form: exact int8 codes x three int8 slices of the FP64 operand,
+// Mixed form: exact int8 codes x two or three int8 slices of the FP64 operand,
 // int32 accumulation; see k_mixed.hip.
 // T = sum of the nseg FP32 partials of T (rows a multiple of 128); with
 // smax non-null also the per-split maxima that launch_uw8 needs (splits of
@@ -115,52 +115,61 @@ hipError_t launch_tprep(hipStream_t st, const float *Tpart, int nseg, int64_t ro
                         unsigned *smax);
 hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int64_t n,
                              double *out, int accumulate);
-// Column tiles of the int8 forms: the three slices of k right-hand columns
-// as 16-wide MFMA tiles.  F whole column tiles take 3 tiles each; the R
-// columns left over (R <= 10) share RT = ceil(3R/16) tiles; R > 10 is padded
-// to a whole tile.  kp() is the padded column count of the FP64 buffers.
+// Column tiles of the int8 forms: the NS slices (2 or 3) of k right-hand
+// columns as 16-wide MFMA tiles.  F whole column tiles take NS tiles each;
+// the R columns left over share RT = ceil(NS R / 16) tiles when that is
+// fewer than NS, else they are padded to a whole tile.  kp() is the padded
+// column count of the FP64 buffers (k rounded up to 16 either way).
 struct ColTiles {
-    int F, R, RT;
-    int vt() const { return 3 * F + RT; }
+    int NS, F, R, RT;
+    int vt() const { return NS * F + RT; }
     int kp() const { return 16 * (F + (RT > 0 ? 1 : 0)); }
 };
-inline ColTiles col_tiles(int k)
+inline ColTiles col_tiles(int k, int ns)
 {
-    ColTiles t{k / 16, k % 16, 0};
-    if (t.R > 10) {
+    ColTiles t{ns, k / 16, k % 16, 0};
+    t.RT = (ns * t.R + 15) / 16;
+    if (t.RT >= ns) {
         t.F += 1;
         t.R = 0;
+        t.RT = 0;
     }
-    t.RT = (3 * t.R + 15) / 16;
     return t;
 }
-// Z^T T on int8 MFMA: U = d T, W = -d mu T as three int8 slices per (SNP
-// split of rps rows, column) with a power-of-two scale from smax.
+// Z^T T on int8 MFMA: U = d T, W = -d mu T as ns int8 slices per (SNP split
+// of rps rows, column) with a power-of-two scale from smax.
 // UW8: rows/8 x vt x 16 words of 16 bytes; smax: ceil(rows/rps) x kp
 // (computed here unless have_max: then already computed by launch_tprep).
-hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int kp,
+hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int ns, int kp,
                       const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8,
                       bool have_max);
 hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const void *UW8, const unsigned *smax, int k, int sa_tiles,
-                        int64_t rows_per_split, int nsplit, float *part);
-// Z Q on int8 MFMA: codes as exact h / m rows, Q as three int8 slices per
+                        const void *UW8, const unsigned *smax, int k, int ns, int sa_tiles,
+                        int waves, int64_t rows_per_split, int nsplit, float *part);
+// Z Q on int8 MFMA: codes as exact h / m rows, Q as ns int8 slices per
 // (sample segment of 4*seg_bytes samples, column) with a power-of-two scale
 // from qmax.  Q8: Mpad/16 x vt x 16 words of 16 bytes; qmax: nseg x kp.
-hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int kp,
+hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int ns, int kp,
                      int64_t seg_bytes, unsigned *qmax, void *Q8);
 hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                         const void *Q8, const unsigned *qmax, const double *mu,
-                        const double *dsc, int k, int st_tiles, int64_t seg_bytes, int nseg,
-                        float *Tpart);
-// tiles per wave (sample tiles of k_zt_i8 for a batch of ldb bytes, SNP
-// tiles of k_za_i8): the accumulators must fit the registers, and one tile
-// is taken where it doubles the workgroups resident per CU (needs the
-// current device; without one the two-tile form)
-int zt_i8_tiles(int k, int64_t ldb);
-int za_i8_tiles(int k);
+                        const double *dsc, int k, int ns, int st_tiles, int waves,
+                        int64_t seg_bytes, int nseg, float *Tpart);
+// Both int8 products run workgroups of `waves` = 8 waves (two per SIMD) or 4
+// (one per SIMD, twice the registers: larger tiles per wave).  Tiles per
+// wave (sample tiles of k_zt_i8 for a batch of ldb bytes, SNP tiles of
+// k_za_i8): the largest form whose accumulators fit the registers, unless a
+// smaller one doubles the workgroups resident per CU (needs the current
+// device; without one the largest form).  *_valid says whether a tile count
+// is instantiated for these columns (the tuning knobs).
+int zt_i8_tiles(int k, int ns, int64_t ldb, int waves);
+int za_i8_tiles(int k, int ns, int waves);
+bool zt_i8_valid(int k, int ns, int sa_tiles, int64_t ldb, int waves);
+bool za_i8_valid(int k, int ns, int st_tiles, int waves);
 // workgroups of k_zt_i8 with sa_tiles resident per CU (0 if unknown)
-int zt_i8_blocks_per_cu(int k, int sa_tiles);
+int zt_i8_blocks_per_cu(int k, int ns, int sa_tiles, int waves);
+// packed bytes per row that one k_zt_i8 workgroup covers
+int64_t zt_i8_tile_bytes(int sa_tiles, int waves);
 
 // Tall-skinny helpers, FP64.  X, Y: m x kp row-major.
 hipError_t launch_gram(hipStream_t st, const double *X, const double *Y, int64_t m, int kp,

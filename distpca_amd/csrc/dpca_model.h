@@ -9,6 +9,7 @@
 struct dpca_model {
     int dev = 0;
     int precision = 0;
+    int ns = 3;                // mixed form: int8 slices of the loadings
     int64_t N = 0, Npad = 0;   // SNPs of the training matrix (rows of a payload)
     int nsv = 0, k = 0, kp = 0;  // k: columns of the training context (its column tiles)
     double *zval = nullptr;    // Npad x 4, training table (FP64 form)

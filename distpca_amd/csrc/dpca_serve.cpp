@@ -46,7 +46,7 @@ ProjPlan plan_projection(const dpca_model *m, int64_t ldb)
         pl.splits = (int)std::max<int64_t>(1, (m->Npad + kProjSplitRowsF64 - 1) / kProjSplitRowsF64);
         pl.part_bytes = (size_t)pl.splits * samples * m->kp * sizeof(double);
     } else {
-        pl.sa_tiles = zt_i8_tiles(m->k, ldb);
+        pl.sa_tiles = zt_i8_tiles(m->k, m->ns, ldb, 8);
         pl.rps = proj_rps_mx(m->Npad);
         pl.nsplit = (int)((m->Npad + pl.rps - 1) / pl.rps);
         pl.part_bytes = (size_t)pl.nsplit * samples * m->kp * sizeof(float);
@@ -64,8 +64,8 @@ hipError_t launch_projection(hipStream_t st, const dpca_model *m, const ProjPlan
                           (double *)part);
         if (e == hipSuccess) e = launch_reduce_parts(st, (const double *)part, pl.splits, n, scores, 0);
     } else {
-        e = launch_zt_i8(st, batch, m->Npad, pl.ldb, m->UW8, m->smax, m->k, pl.sa_tiles, pl.rps,
-                         pl.nsplit, (float *)part);
+        e = launch_zt_i8(st, batch, m->Npad, pl.ldb, m->UW8, m->smax, m->k, m->ns, pl.sa_tiles, 8,
+                         pl.rps, pl.nsplit, (float *)part);
         if (e == hipSuccess) e = launch_reduce_f32(st, (const float *)part, pl.nsplit, n, scores, 0);
     }
     return e;

@@ -2,14 +2,14 @@
 // (v_mfma_i32_16x16x64_i8).  The genotype codes enter the products as exact
 // int8 values -- h, the allele count (0 when missing), and m, the
 // non-missing indicator -- with Z_si = d_s (h_si - mu_s m_si); the FP64
-// right-hand operand is split into three int8 fixed-point slices with one
-// power-of-two scale per column and group of rows, so every product sums
-// exactly in int32 and only the final combination rounds.
+// right-hand operand is split into NS int8 fixed-point slices (NS = 2 or 3)
+// with one power-of-two scale per column and group of rows, so every
+// product sums exactly in int32 and only the final combination rounds.
 //
-//   T = Z Q      two int8 rows per SNP (h and m) against three slices of Q
-//                (scale per sample segment); T_s = d_s (H_s - mu_s M_s) in
-//                FP64 per segment, FP32 partials, segments summed in FP64.
-//   Z^T T        two int8 K slots per SNP (h and m) against three slices of
+//   T = Z Q      two int8 rows per SNP (centred h and m) against the slices
+//                of Q (scale per sample segment); T_s in FP64 per segment,
+//                FP32 partials, segments summed in FP64.
+//   Z^T T        two int8 K slots per SNP (h and m) against the slices of
 //                U = d T and W = -d mu T (scale per SNP split); splits
 //                summed in FP64.
 #include "dpca_internal.h"
@@ -86,20 +86,21 @@ hipError_t launch_reduce_f32(hipStream_t st, const float *part, int splits, int6
 }
 
 // ============================================== int8 slices and their tiles ==
-// An FP64 operand x (Q, or U = d T and W = -d mu T) enters the MFMA as three
+// An FP64 operand x (Q, or U = d T and W = -d mu T) enters the MFMA as NS
 // int8 slices of a fixed-point form with one power-of-two scale s per column
 // and group of rows (one workgroup's accumulation range):
-// x = s (q0 + q1 2^-7 + q2 2^-14) + e, |e| <= s 2^-15, s >= max|x| / 127.
-// The slice products sum exactly in int32; the slices are combined in FP64
-// when the group ends.
+// x = s sum_{j<NS} q_j 2^-7j + e, |e| <= s 2^-(7 NS - 6), s >= max|x| / 127
+// (NS = 3: |e| <= s 2^-15; NS = 2: s 2^-8).  The slice products sum exactly
+// in int32; the slices are combined in FP64 when the group ends.
 //
-// Column tiles.  The three slices of the k right-hand columns are laid out
-// as 16-wide MFMA B tiles ("virtual tiles", VT = 3F + RT of them): columns
-// 16c..16c+15, c < F, take three tiles each (tile 3c + j holds slice j in
-// lane c%16); the R = k - 16F columns left over (R <= 10) share RT =
-// ceil(3R / 16) tiles: virtual column w = jR + x holds slice j of column
-// 16F + x, in tile 3F + w/16, lane w%16.  So k = 40 takes 8 tiles, not 9,
-// and k = 20 four, not six.  k with R > 10 is padded to whole tiles.
+// Column tiles.  The NS slices of the k right-hand columns are laid out as
+// 16-wide MFMA B tiles ("virtual tiles", VT = NS F + RT of them): columns
+// 16c..16c+15, c < F, take NS tiles each (tile NS c + j holds slice j in
+// lane c%16); the R = k - 16F columns left over share RT = ceil(NS R / 16)
+// tiles: virtual column w = jR + x holds slice j of column 16F + x, in tile
+// NS F + w/16, lane w%16.  So k = 40 takes 8 tiles with 3 slices (not 9)
+// and exactly 5 with 2.  Remainders that would need NS tiles anyway are
+// padded to whole tiles.
 //
 // MFMA operand layout (v_mfma_i32_16x16x64_i8, checked on the device): lane
 // l = 16g + r holds A[r][16g..16g+15] and B[16g..16g+15][r] as bytes and
@@ -122,36 +123,37 @@ __device__ __forceinline__ double slice_scale(float mx)
     return ldexp(1.0, e);
 }
 
-// |x| <= 127 s: three int8 slices of x / s, each residual scaled by 2^7.
-__device__ __forceinline__ void slices3(double x, double inv_s, int q[3])
+// |x| <= 127 s: NS int8 slices of x / s, each residual scaled by 2^7.
+template <int NS>
+__device__ __forceinline__ void slices(double x, double inv_s, int (&q)[NS])
 {
     double v = x * inv_s;              // exact: inv_s is a power of two
-    const double a = rint(v);
-    v = (v - a) * 128.0;               // exact, |v| <= 64
-    const double b = rint(v);
-    v = (v - b) * 128.0;
-    q[0] = (int)a;
-    q[1] = (int)b;
-    q[2] = (int)rint(v);
+#pragma unroll
+    for (int j = 0; j < NS; j++) {
+        const double a = rint(v);
+        q[j] = (int)a;
+        v = (v - a) * 128.0;           // exact, |v| <= 64
+    }
 }
 
 // Slice j and column of virtual column (tile v, lane r); column -1 = empty.
+template <int NS>
 __device__ __forceinline__ void vcol(int v, int r, int F, int R, int &j, int &col)
 {
-    if (v < 3 * F) {
-        j = v % 3;
-        col = (v / 3) * 16 + r;
+    if (v < NS * F) {
+        j = v % NS;
+        col = (v / NS) * 16 + r;
         return;
     }
-    const int w = (v - 3 * F) * 16 + r;
-    j = R ? w / R : 3;
-    col = (j < 3) ? 16 * F + w % R : -1;
+    const int w = (v - NS * F) * 16 + r;
+    j = R ? w / R : NS;
+    col = (j < NS) ? 16 * F + w % R : -1;
 }
 
 // Value of slice j of column 16F + x for this lane's D rows: the remainder
 // tiles' accumulators are spread over lanes 16g + (jR + x) % 16.
-template <int F, int RT>
-__device__ __forceinline__ int rem_slice(const i32x4 (&acc)[3 * F + RT], int q, int j, int x,
+template <int NS, int F, int RT>
+__device__ __forceinline__ int rem_slice(const i32x4 (&acc)[NS * F + RT], int q, int j, int x,
                                          int R, int g)
 {
     const int w = j * R + x;
@@ -159,42 +161,31 @@ __device__ __forceinline__ int rem_slice(const i32x4 (&acc)[3 * F + RT], int q, 
     int val = 0;
 #pragma unroll
     for (int u = 0; u < RT; u++) {
-        const int y = __shfl(acc[3 * F + u][q], src, 64);
+        const int y = __shfl(acc[NS * F + u][q], src, 64);
         if ((w >> 4) == u) val = y;
     }
     return val;
 }
 
-// Combined FP64 value sum_j 2^-7j slice_j of output column 16 oc + r for
-// D row 4g + q (zero for padding columns).
-template <int F, int RT>
-__device__ __forceinline__ double combine(const i32x4 (&acc)[3 * F + RT], int oc, int q, int r,
+// Combined FP64 value sum_{j<NJ} 2^-7j slice_j of output column 16 oc + r for
+// D row 4g + q (zero for padding columns).  NJ = NS combines every slice;
+// NJ = NS - 1 leaves out the last (a product that needs 2^-7 less precision
+// does not compute the tiles that hold only the last slice).
+template <int NS, int NJ, int F, int RT>
+__device__ __forceinline__ double combine(const i32x4 (&acc)[NS * F + RT], int oc, int q, int r,
                                           int R, int g)
 {
-    if (oc < F)
-        return (double)acc[3 * oc][q] + ldexp((double)acc[3 * oc + 1][q], -7) +
-               ldexp((double)acc[3 * oc + 2][q], -14);
+    double v = 0.0;
+    if (oc < F) {
+#pragma unroll
+        for (int j = 0; j < NJ; j++) v += ldexp((double)acc[NS * oc + j][q], -7 * j);
+        return v;
+    }
     // remainder columns: every lane joins the shuffles, lanes r >= R drop out
     const int x = r < R ? r : 0;
-    const int s0 = rem_slice<F, RT>(acc, q, 0, x, R, g);
-    const int s1 = rem_slice<F, RT>(acc, q, 1, x, R, g);
-    const int s2 = rem_slice<F, RT>(acc, q, 2, x, R, g);
-    if (r >= R) return 0.0;
-    return (double)s0 + ldexp((double)s1, -7) + ldexp((double)s2, -14);
-}
-
-// Same from slices 0 and 1 only (the slice-2 tiles of a product that needs
-// 2^-7 less precision are not computed).
-template <int F, int RT>
-__device__ __forceinline__ double combine2(const i32x4 (&acc)[3 * F + RT], int oc, int q, int r,
-                                           int R, int g)
-{
-    if (oc < F) return (double)acc[3 * oc][q] + ldexp((double)acc[3 * oc + 1][q], -7);
-    const int x = r < R ? r : 0;
-    const int s0 = rem_slice<F, RT>(acc, q, 0, x, R, g);
-    const int s1 = rem_slice<F, RT>(acc, q, 1, x, R, g);
-    if (r >= R) return 0.0;
-    return (double)s0 + ldexp((double)s1, -7);
+#pragma unroll
+    for (int j = 0; j < NJ; j++) v += ldexp((double)rem_slice<NS, F, RT>(acc, q, j, x, R, g), -7 * j);
+    return r >= R ? 0.0 : v;
 }
 
 // smax[split * kp + c] = max over rows of the split of f_s |T_sc|, with
@@ -239,6 +230,7 @@ __global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
 // word {U_j(8G..+3), U_j(8G+4..+7), W_j(8G..+3), W_j(8G+4..+7)} of the slice
 // and column of (v, r), at UW8[(G * VT + v) * 16 + r].
 
+template <int NS>
 __global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F, int R, int VT,
                       const double *__restrict__ mu, const double *__restrict__ dsc, int64_t rps,
                       const unsigned *__restrict__ smax, u32x4 *__restrict__ UW8)
@@ -247,7 +239,7 @@ __global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F,
         const int64_t G = idx / (VT * 16);
         const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
         int j, c;
-        vcol(v, r, F, R, j, c);
+        vcol<NS>(v, r, F, R, j, c);
         uint32_t w[4] = {0, 0, 0, 0};
         if (c >= 0) {
             const int64_t split = (G * 8) / rps;
@@ -256,9 +248,9 @@ __global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F,
             for (int e = 0; e < 8; e++) {
                 const int64_t row = G * 8 + e;
                 const double t = T[row * kp + c];
-                int qu[3], qw[3];
-                slices3(dsc[row] * t, inv, qu);
-                slices3(-(dsc[row] * mu[row]) * t, inv, qw);
+                int qu[NS], qw[NS];
+                slices<NS>(dsc[row] * t, inv, qu);
+                slices<NS>(-(dsc[row] * mu[row]) * t, inv, qw);
                 const int sh = 8 * (e & 3);
                 w[e >> 2] |= (uint32_t)(qu[j] & 0xFF) << sh;
                 w[2 + (e >> 2)] |= (uint32_t)(qw[j] & 0xFF) << sh;
@@ -268,11 +260,12 @@ __global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F,
     }
 }
 
-hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int kp,
+hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int ns, int kp,
                       const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8,
                       bool have_max)
 {
-    const ColTiles ct = col_tiles(k);
+    if (ns != 2 && ns != 3) return hipErrorInvalidValue;
+    const ColTiles ct = col_tiles(k, ns);
     if (rows % 128 || rps % 128 || rps < 128 || kp * 4 > 256 || ct.kp() != kp)
         return hipErrorInvalidValue;
     const int64_t nsplit = (rows + rps - 1) / rps;
@@ -284,27 +277,32 @@ hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int 
         k_split_max<<<g1, 4 * kp, 0, st>>>(T, rows, kp, mu, dsc, rps, smax);
     }
     const int64_t n = rows / 8 * ct.vt() * 16;
-    k_uw8<<<ew_blocks(n), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc,
-                                                        rps, smax, reinterpret_cast<u32x4 *>(UW8));
+    if (ns == 2)
+        k_uw8<2><<<ew_blocks(n), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc, rps, smax,
+                                                reinterpret_cast<u32x4 *>(UW8));
+    else
+        k_uw8<3><<<ew_blocks(n), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc, rps, smax,
+                                                reinterpret_cast<u32x4 *>(UW8));
     return hipGetLastError();
 }
 
-// Workgroup: 8 waves x SA tiles x 16 samples (SA = 4: 512 samples = 128
+// Workgroup: WAVES waves x SA tiles x 16 samples (8 x 4: 512 samples = 128
 // packed bytes per row) and one SNP split (blockIdx.y); 128-SNP chunks of
 // packed bytes staged in LDS transposed to [byte column][4-SNP group], so a
 // lane reads the codes of 4 consecutive SNPs for its sample with one dword
 // load, and the chunk's UW8 words copied to LDS.
-template <int SA, int F, int RT>
-__global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packed, int64_t ldp,
+template <int NS, int SA, int F, int RT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ UW8,
                                                const unsigned *__restrict__ smax, int64_t rows,
                                                int64_t rows_per_split, int64_t Mpad, int R,
                                                float *__restrict__ part)
 {
-    constexpr int VT = 3 * F + RT;           // virtual column tiles
+    constexpr int VT = NS * F + RT;          // virtual column tiles
     constexpr int CT = F + (RT > 0 ? 1 : 0); // output column tiles
     constexpr int KP = CT * 16;
-    constexpr int TB = SA * 16 * 8 / 4;    // packed bytes per row in this tile
+    constexpr int NT = 64 * WAVES;         // threads
+    constexpr int TB = SA * 16 * WAVES / 4;  // packed bytes per row in this tile
     constexpr int SC = 128;                // SNPs per chunk
     constexpr int NUWT = (SC / 8) * VT * 16;  // UW8 words per chunk
     // two buffers: chunk i is multiplied out of one while chunk i+1 lands in
@@ -332,9 +330,9 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
 #pragma unroll
         for (int v = 0; v < VT; v++) acc[a][v] = i32x4{0, 0, 0, 0};
 
-    constexpr int NPK = (SC / 4) * (TB / 4) / 512;   // 4x4 byte blocks per thread
-    constexpr int NUW = (NUWT + 511) / 512;          // UW8 words per thread
-    static_assert((SC / 4) * (TB / 4) % 512 == 0, "tile");
+    constexpr int NPK = (SC / 4) * (TB / 4) / NT;   // 4x4 byte blocks per thread
+    constexpr int NUW = (NUWT + NT - 1) / NT;       // UW8 words per thread
+    static_assert((SC / 4) * (TB / 4) % NT == 0, "tile");
     // software pipeline: the next chunk is loaded into registers while the
     // current one is multiplied, then stored into the other LDS buffer
     uint32_t rpk[NPK][4];
@@ -342,7 +340,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
 #define DPCA_ZT8_LOADP(c0, dst)                                                             \
     do {                                                                                    \
         _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
-            const int it_ = tid + 512 * u;                                                  \
+            const int it_ = tid + NT * u;                                                   \
             const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
             const uint8_t *p_ = packed + ((c0) + 4 * sg_) * ldp + mb + 4 * bq_;             \
             _Pragma("unroll") for (int j = 0; j < 4; j++) dst[u][j] =                      \
@@ -353,7 +351,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
     do {                                                                                    \
         const u32x4 *q_ = UW8 + ((c0) / 8) * VT * 16;                                       \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
-            const int it_ = tid + 512 * u;                                                  \
+            const int it_ = tid + NT * u;                                                   \
             if (it_ < NUWT) ruw[u] = q_[it_];                                               \
         }                                                                                   \
     } while (0)
@@ -361,7 +359,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
 #define DPCA_ZT8_STORE(buf)                                                                 \
     do {                                                                                    \
         _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
-            const int it_ = tid + 512 * u;                                                  \
+            const int it_ = tid + NT * u;                                                   \
             const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
             const uint32_t r0_ = rpk[u][0], r1_ = rpk[u][1], r2_ = rpk[u][2], r3_ = rpk[u][3]; \
             const uint32_t lo01_ = __builtin_amdgcn_perm(r1_, r0_, 0x05010400u);            \
@@ -375,7 +373,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             pk[buf][4 * bq_ + 3][col_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x07060302u);  \
         }                                                                                   \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
-            const int it_ = tid + 512 * u;                                                  \
+            const int it_ = tid + NT * u;                                                   \
             if (it_ < NUWT) uw[buf][it_] = ruw[u];                                          \
         }                                                                                   \
     } while (0)
@@ -395,29 +393,61 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
             DPCA_ZT8_LOADP(nx, rpk);
             DPCA_ZT8_LOADU(nx);
         }
+        // The chunk's B tiles (K step t, tile v) form one sequence i = t VT + v
+        // read from LDS ZB tiles ahead of their MFMAs, and the packed words
+        // of K step t + 1 are read while step t multiplies, so the LDS
+        // latency hides behind the matrix cores instead of stalling them.
+        constexpr int NKS = SC / 32;            // K steps per chunk
+        constexpr int NB = NKS * VT;            // B tiles per chunk
+        constexpr int ZB = 4;                   // B tiles in flight
+        const u32x4 *ub = &uw[cur][g * VT * 16 + r];
+        auto boff = [](int i) constexpr { return ((i / VT) * 4 * VT + i % VT) * 16; };
+        u32x4 bq[ZB];
 #pragma unroll
-        for (int t = 0; t < SC / 32; t++) {
-            u32x4 av[SA];
+        for (int i = 0; i < ZB; i++) bq[i] = ub[boff(i)];
+        uint2 wq[SA];
+        auto read_words = [&](int t) {
 #pragma unroll
             for (int a = 0; a < SA; a++) {
                 const int i = wave * SA * 16 + a * 16 + r;  // sample within the tile
-                const int sh = 2 * (i & 3);
                 const int sw = (wave * SA + a) & 30;         // (i / 16) & ~1
-                const uint2 w2 = *reinterpret_cast<const uint2 *>(
-                    &pk[cur][i >> 2][(8 * t + 2 * g) ^ sw]);
-                const uint32_t x0 = (a & 1) ? w2.y : w2.x;   // group 8t + 2g
-                const uint32_t x1 = (a & 1) ? w2.x : w2.y;   // group 8t + 2g + 1
-                const uint32_t c0 = (x0 >> sh) & 0x03030303u;
-                const uint32_t c1 = (x1 >> sh) & 0x03030303u;
-                av[a] = u32x4{__builtin_amdgcn_perm(0u, HT, c0), __builtin_amdgcn_perm(0u, HT, c1),
-                              __builtin_amdgcn_perm(0u, MT, c0), __builtin_amdgcn_perm(0u, MT, c1)};
+                wq[a] = *reinterpret_cast<const uint2 *>(&pk[cur][i >> 2][(8 * t + 2 * g) ^ sw]);
             }
+        };
+        // A operands of K step t + 1 are built from its packed words (read at
+        // the start of step t) while step t multiplies, one sample tile per B
+        // tile from the third on (the last tile when there are fewer), so the
+        // VALU work interleaves with the MFMAs
+        u32x4 av[2][SA];
+        auto build = [&](int t, int a) {
+            const int sh = 2 * (r & 3);                      // sample i & 3 = r & 3
+            const uint32_t x0 = (a & 1) ? wq[a].y : wq[a].x;   // group 8t + 2g
+            const uint32_t x1 = (a & 1) ? wq[a].x : wq[a].y;   // group 8t + 2g + 1
+            const uint32_t c0 = (x0 >> sh) & 0x03030303u;
+            const uint32_t c1 = (x1 >> sh) & 0x03030303u;
+            av[t & 1][a] = u32x4{__builtin_amdgcn_perm(0u, HT, c0), __builtin_amdgcn_perm(0u, HT, c1),
+                                 __builtin_amdgcn_perm(0u, MT, c0), __builtin_amdgcn_perm(0u, MT, c1)};
+        };
+        read_words(0);
 #pragma unroll
-            for (int v = 0; v < VT; v++) {
-                const u32x4 b = uw[cur][((4 * t + g) * VT + v) * 16 + r];
+        for (int a = 0; a < SA; a++) build(0, a);
 #pragma unroll
-                for (int a = 0; a < SA; a++) acc[a][v] = mfma_i8(av[a], b, acc[a][v]);
+        for (int i = 0; i < NB; i++) {
+            const int t = i / VT, v = i % VT;
+            if (v == 0 && t + 1 < NKS) read_words(t + 1);
+            const u32x4 b = bq[i % ZB];
+            if (i + ZB < NB) bq[i % ZB] = ub[boff(i + ZB)];
+#pragma unroll
+            for (int a = 0; a < SA; a++) acc[a][v] = mfma_i8(av[t & 1][a], b, acc[a][v]);
+            if (t + 1 < NKS) {
+#pragma unroll
+                for (int a = 0; a < SA; a++)
+                    if ((VT >= 3 ? 2 + a * (VT - 2) / SA : VT - 1) == v) build(t + 1, a);
             }
+            // keep each tile's LDS reads, MFMAs and A piece together (the
+            // scheduler would otherwise sink the reads next to their use and
+            // bunch the VALU work); only SALU moves across
+            __builtin_amdgcn_sched_barrier(0x4);
         }
         DPCA_ZT8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
         __syncthreads();
@@ -435,7 +465,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
 #pragma unroll
             for (int q = 0; q < 4; q++)
                 out[(a * 16 + 4 * g + q) * KP + col] =
-                    (float)(s * combine<F, RT>(acc[a], oc, q, r, R, g));
+                    (float)(s * combine<NS, NS, F, RT>(acc[a], oc, q, r, R, g));
     }
 }
 
@@ -452,6 +482,7 @@ __global__ __launch_bounds__(512) void k_zt_i8(const uint8_t *__restrict__ packe
 // byte e holds sample 16G + 4 (e & 3) + (e >> 2), the order in which k_za_i8
 // unpacks a packed dword (its dword q holds the bit pairs q of the four
 // bytes); qmax[seg * kp + c] the column's max |Q| over the segment.
+template <int NS>
 __global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int F, int R, int VT,
                      int64_t seg_samples, const unsigned *__restrict__ qmax,
                      u32x4 *__restrict__ Q8)
@@ -460,15 +491,15 @@ __global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int F, 
         const int64_t G = idx / (VT * 16);
         const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
         int j, c;
-        vcol(v, r, F, R, j, c);
+        vcol<NS>(v, r, F, R, j, c);
         uint32_t w[4] = {0, 0, 0, 0};
         if (c >= 0) {
             const int64_t seg = (G * 16) / seg_samples;
             const double inv = 1.0 / slice_scale(__uint_as_float(qmax[seg * kp + c]));
 #pragma unroll
             for (int e = 0; e < 16; e++) {
-                int q[3];
-                slices3(Q[(G * 16 + 4 * (e & 3) + (e >> 2)) * kp + c], inv, q);
+                int q[NS];
+                slices<NS>(Q[(G * 16 + 4 * (e & 3) + (e >> 2)) * kp + c], inv, q);
                 w[e >> 2] |= (uint32_t)(q[j] & 0xFF) << (8 * (e & 3));
             }
         }
@@ -476,10 +507,11 @@ __global__ void k_q8(const double *__restrict__ Q, int64_t Mpad, int kp, int F, 
     }
 }
 
-hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int kp,
+hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int ns, int kp,
                      int64_t seg_bytes, unsigned *qmax, void *Q8)
 {
-    const ColTiles ct = col_tiles(k);
+    if (ns != 2 && ns != 3) return hipErrorInvalidValue;
+    const ColTiles ct = col_tiles(k, ns);
     const int64_t segs = 4 * seg_bytes;  // samples per segment
     if (Mpad % 256 || segs % 256 || kp * 4 > 256 || ct.kp() != kp) return hipErrorInvalidValue;
     const int64_t nseg = (Mpad + segs - 1) / segs;
@@ -488,24 +520,28 @@ hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int k
     dim3 g1((unsigned)((segs + 127) / 128), (unsigned)nseg);
     k_split_max<<<g1, 4 * kp, 0, st>>>(Q, Mpad, kp, nullptr, nullptr, segs, qmax);
     const int64_t n = Mpad / 16 * ct.vt() * 16;
-    k_q8<<<ew_blocks(n), 256, 0, st>>>(Q, Mpad, kp, ct.F, ct.R, ct.vt(), segs, qmax,
-                                                       reinterpret_cast<u32x4 *>(Q8));
+    if (ns == 2)
+        k_q8<2><<<ew_blocks(n), 256, 0, st>>>(Q, Mpad, kp, ct.F, ct.R, ct.vt(), segs, qmax,
+                                               reinterpret_cast<u32x4 *>(Q8));
+    else
+        k_q8<3><<<ew_blocks(n), 256, 0, st>>>(Q, Mpad, kp, ct.F, ct.R, ct.vt(), segs, qmax,
+                                               reinterpret_cast<u32x4 *>(Q8));
     return hipGetLastError();
 }
 
-// Workgroup: 8 waves x ST tiles x 16 SNPs, one sample segment (blockIdx.y).
+// Workgroup: WAVES waves x ST tiles x 16 SNPs, one sample segment (blockIdx.y).
 // Per chunk of CB packed bytes per row (4 CB samples), lane group g owns the
 // CB samples of its CB / 4 packed bytes; at K step t it supplies samples
 // CB g + 16t .. +15.  The chunk's Q8 words are staged in LDS.
-template <int ST, int F, int RT, bool SKR>
-__global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packed, int64_t ldp,
+template <int NS, int ST, int F, int RT, bool SKR, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ Q8,
                                                const unsigned *__restrict__ qmax,
                                                const double *__restrict__ mu,
                                                const double *__restrict__ dsc, int64_t seg_bytes,
                                                int64_t Npad, int R, float *__restrict__ Tpart)
 {
-    constexpr int VT = 3 * F + RT;
+    constexpr int VT = NS * F + RT;
     constexpr int CT = F + (RT > 0 ? 1 : 0);
     constexpr int KP = CT * 16;
     // packed bytes per chunk: 128 where two LDS buffers of its Q8 words
@@ -513,11 +549,12 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     constexpr int CB = (VT <= 10) ? 128 : 64;
     constexpr int NW = CB / 64;                 // 16-byte words per lane and row
     constexpr int NQT = (CB / 4) * VT * 16;     // Q8 words per chunk
-    constexpr int NQ = (NQT + 511) / 512;
+    constexpr int NT = 64 * WAVES;              // threads
+    constexpr int NQ = (NQT + NT - 1) / NT;
     __shared__ u32x4 qs[2][NQT];               // double buffer, one barrier per chunk
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
-    const int64_t base = (int64_t)blockIdx.x * (8 * ST * 16) + wave * (ST * 16);
+    const int64_t base = (int64_t)blockIdx.x * (WAVES * ST * 16) + wave * (ST * 16);
     const int64_t b0 = (int64_t)blockIdx.y * seg_bytes;
     int64_t b1 = b0 + seg_bytes;
     if (b1 > ldp) b1 = ldp;
@@ -525,7 +562,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     // Per-row A values 63 h - mh with mh = rint(63 mu) (exact int8, 0 when
     // missing), as a byte table by 2-bit code: T_s = d_s (sum (63 h - mh m) Q
     // / 63 - delta_s sum m Q), delta_s = mu_s - mh / 63, |delta_s| <= 1/126,
-    // so the mask product needs only slices 0 and 1 (2^-7 less precision).
+    // so the mask product needs one slice fewer (2^-7 less precision).
     uint32_t tab[ST];
 #pragma unroll
     for (int s = 0; s < ST; s++) {
@@ -533,10 +570,10 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
         tab[s] = ((uint32_t)(-mh) & 0xFFu) | (((uint32_t)(63 - mh) & 0xFFu) << 16) |
                  (((uint32_t)(126 - mh) & 0xFFu) << 24);
     }
-    // tiles holding only slice 2 (of whole columns, and the last remainder
-    // tile when SKR) are left out of the mask product
-    auto slice2_only = [](int v) constexpr {
-        return v < 3 * F ? v % 3 == 2 : (SKR && v == 3 * F + RT - 1);
+    // tiles holding only the last slice (of whole columns, and the last
+    // remainder tile when SKR) are left out of the mask product
+    auto last_only = [](int v) constexpr {
+        return v < NS * F ? v % NS == NS - 1 : (SKR && v == NS * F + RT - 1);
     };
 
     i32x4 acch[ST][VT], accm[ST][VT];
@@ -556,7 +593,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     do {                                                                                    \
         const u32x4 *src_ = Q8 + ((c0) / 4) * VT * 16;                                      \
         _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
-            const int it_ = tid + 512 * u;                                                  \
+            const int it_ = tid + NT * u;                                                   \
             if (it_ < NQT) rq[u] = src_[it_];                                               \
         }                                                                                   \
         _Pragma("unroll") for (int s = 0; s < ST; s++)                                     \
@@ -567,7 +604,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 #define DPCA_ZA8_STORE(buf)                                                                 \
     do {                                                                                    \
         _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
-            const int it_ = tid + 512 * u;                                                  \
+            const int it_ = tid + NT * u;                                                   \
             if (it_ < NQT) qs[buf][it_] = rq[u];                                            \
         }                                                                                   \
     } while (0)
@@ -575,7 +612,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
     DPCA_ZA8_STORE(0);
     // where the registers allow, the packed codes (HBM) run two chunks
     // ahead and the Q8 words (L2) one
-    constexpr bool PF2 = ST * VT <= 16;
+    constexpr bool PF2 = ST * VT < 16;
     u32x4 rp2[ST][NW];
     if constexpr (PF2) {
         const int64_t c2 = (b0 + CB < b1) ? b0 + CB : b0;
@@ -603,7 +640,7 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
                 const u32x4 *src_ = Q8 + (nx / 4) * VT * 16;
 #pragma unroll
                 for (int u = 0; u < NQ; u++) {
-                    const int it_ = tid + 512 * u;
+                    const int it_ = tid + NT * u;
                     if (it_ < NQT) rq[u] = src_[it_];
                 }
 #pragma unroll
@@ -618,31 +655,51 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
                 DPCA_ZA8_LOAD(nx);
             }
         }
+        // The chunk's B tiles (K step t, tile v) form one sequence i = t VT + v
+        // read from LDS ZB tiles ahead of their MFMAs (the LDS latency then
+        // hides behind the matrix cores instead of stalling them).
+        constexpr int NB = 4 * NW * VT;         // B tiles per chunk
+        constexpr int ZB = 4;                   // B tiles in flight
+        const u32x4 *qb = &qs[cur][(CB / 16) * g * VT * 16 + r];
+        u32x4 bq[ZB];
 #pragma unroll
-        for (int t = 0; t < 4 * NW; t++) {
-            // dword q of the A operand: bit pair q of each of the four
-            // packed bytes, i.e. samples 4b + q for b = 0..3 (Q8 bytes follow
-            // the same order), one shift and mask per four codes
-            u32x4 ah[ST], am[ST];
+        for (int i = 0; i < ZB; i++) bq[i] = qb[i * 16];
+        // A operands of K step t + 1 are built while step t multiplies: its
+        // 4 ST (tile, dword) pieces are spread over the step's B tiles, so
+        // the VALU work interleaves with the MFMAs of the same wave.
+        // dword q of an A operand: bit pair q of each of the four packed
+        // bytes, i.e. samples 4b + q for b = 0..3 (Q8 bytes follow the same
+        // order), one shift and mask per four codes.
+        constexpr int NKS = 4 * NW;             // K steps per chunk
+        constexpr int NP = 4 * ST;              // A pieces per K step
+        u32x4 ah[2][ST], am[2][ST];
+        auto build = [&](int t, int p) {
+            const int s = p / 4, q = p % 4;
+            const uint32_t cw = (pk[s][t / 4][t % 4] >> (2 * q)) & 0x03030303u;
+            ah[t & 1][s][q] = __builtin_amdgcn_perm(0u, tab[s], cw);
+            am[t & 1][s][q] = __builtin_amdgcn_perm(0u, MT, cw);
+        };
+#pragma unroll
+        for (int p = 0; p < NP; p++) build(0, p);
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+            const int t = i / VT, v = i % VT;
+            const u32x4 b = bq[i % ZB];
+            if (i + ZB < NB) bq[i % ZB] = qb[(i + ZB) * 16];
 #pragma unroll
             for (int s = 0; s < ST; s++) {
-                const uint32_t w = pk[s][t / 4][t % 4];
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const uint32_t cw = (w >> (2 * q)) & 0x03030303u;
-                    ah[s][q] = __builtin_amdgcn_perm(0u, tab[s], cw);
-                    am[s][q] = __builtin_amdgcn_perm(0u, MT, cw);
-                }
+                acch[s][v] = mfma_i8(ah[t & 1][s], b, acch[s][v]);
+                if (!last_only(v)) accm[s][v] = mfma_i8(am[t & 1][s], b, accm[s][v]);
             }
+            if (t + 1 < NKS) {
 #pragma unroll
-            for (int v = 0; v < VT; v++) {
-                const u32x4 b = qs[cur][(((CB / 16) * g + t) * VT + v) * 16 + r];
-#pragma unroll
-                for (int s = 0; s < ST; s++) {
-                    acch[s][v] = mfma_i8(ah[s], b, acch[s][v]);
-                    if (!slice2_only(v)) accm[s][v] = mfma_i8(am[s], b, accm[s][v]);
-                }
+                for (int p = 0; p < NP; p++)
+                    if (p * VT / NP == v) build(t + 1, p);
             }
+            // keep each tile's LDS read, MFMAs and A piece together (the
+            // scheduler would otherwise sink the reads next to their use and
+            // bunch the VALU work); only SALU moves across
+            __builtin_amdgcn_sched_barrier(0x4);
         }
         DPCA_ZA8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
         __syncthreads();
@@ -659,8 +716,8 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 const int64_t row = base + s * 16 + 4 * g + q;
-                const double a = combine<F, RT>(acch[s], oc, q, r, R, g);
-                const double m = combine2<F, RT>(accm[s], oc, q, r, R, g);
+                const double a = combine<NS, NS, F, RT>(acch[s], oc, q, r, R, g);
+                const double m = combine<NS, NS - 1, F, RT>(accm[s], oc, q, r, R, g);
                 const double mr = mu[row];
                 const double delta = mr - rint(63.0 * mr) / 63.0;
                 out[row * KP + col] = (float)(dsc[row] * sc * (a / 63.0 - delta * m));
@@ -669,182 +726,261 @@ __global__ __launch_bounds__(512) void k_za_i8(const uint8_t *__restrict__ packe
 }
 
 // ------------------------------------------------------------- launchers --
-// Instantiated tile shapes: (F, RT) of col_tiles(k) for k = 1..64, with the
-// sample / SNP tiles per wave chosen so the accumulators fit the registers.
-#define DPCA_I8_SHAPES(X) \
-    X(1, 0) X(2, 0) X(3, 0) X(4, 0) X(0, 1) X(1, 1) X(2, 1) X(3, 1) X(0, 2) X(1, 2) X(2, 2) X(3, 2)
+// Instantiated tile shapes: (NS, F, RT) of col_tiles(k, NS) for k = 1..64 and
+// NS = 3 or 2, with the sample / SNP tiles per wave whose accumulators,
+// double-buffered A operands and prefetch registers fit without spilling
+// (zt_fits / za_fits, checked against the compiler's register report).
+#define DPCA_I8_SHAPES(X)                                                                  \
+    X(3, 1, 0) X(3, 2, 0) X(3, 3, 0) X(3, 4, 0) X(3, 0, 1) X(3, 1, 1) X(3, 2, 1) X(3, 3, 1)   \
+        X(3, 0, 2) X(3, 1, 2) X(3, 2, 2) X(3, 3, 2) X(2, 1, 0) X(2, 2, 0) X(2, 3, 0)          \
+            X(2, 4, 0) X(2, 0, 1) X(2, 1, 1) X(2, 2, 1) X(2, 3, 1)
 
-template <int SA, int F, int RT>
+// Workgroups of 8 waves (two per SIMD, 256 registers each) or 4 (one per
+// SIMD with up to 512 registers, the accumulators in AGPRs: twice the tile
+// per wave, so each LDS operand word feeds twice the MFMAs; measured slower
+// at 100k x 1M, kept as a tuning form).
+constexpr bool zt_fits(int sa, int vt, int waves)
+{
+    return waves == 8 ? sa <= 4 && sa * vt <= 32 : sa * vt <= 48;
+}
+constexpr bool za_fits(int st, int vt, int waves)
+{
+    return waves == 8 ? st <= 2 && st * vt <= 16 : st * vt <= 24;
+}
+// packed bytes per row of one Z^T T workgroup's sample tile
+constexpr int zt_tile_bytes(int sa, int waves) { return 4 * sa * waves; }
+
+template <int NS, int SA, int F, int RT, int W>
 static hipError_t zt_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
                         const u32x4 *UW8, const unsigned *smax, int64_t rows, int64_t rps,
                         int64_t Mpad, int R, float *part)
 {
-    if constexpr (SA * (3 * F + RT) > 36) {
+    if constexpr (!zt_fits(SA, NS * F + RT, W)) {
         return hipErrorInvalidValue;  // would not fit the registers
     } else {
-        k_zt_i8<SA, F, RT><<<grid, 512, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad, R, part);
+        k_zt_i8<NS, SA, F, RT, W><<<grid, 64 * W, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad,
+                                                          R, part);
         return hipGetLastError();
     }
 }
 
-template <int ST, int F, int RT, bool SKR>
+template <int NS, int ST, int F, int RT, bool SKR, int W>
 static hipError_t za_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
                         const u32x4 *Q8, const unsigned *qmax, const double *mu,
                         const double *dsc, int64_t seg_bytes, int64_t rows, int R, float *Tpart)
 {
-    if constexpr (ST * (3 * F + RT) > 18) {
+    if constexpr (!za_fits(ST, NS * F + RT, W)) {
         return hipErrorInvalidValue;  // would not fit the registers
     } else {
-        k_za_i8<ST, F, RT, SKR><<<grid, 512, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc, seg_bytes, rows,
-                                                 R, Tpart);
+        k_za_i8<NS, ST, F, RT, SKR, W><<<grid, 64 * W, 0, st>>>(packed, ldp, Q8, qmax, mu, dsc,
+                                                               seg_bytes, rows, R, Tpart);
         return hipGetLastError();
     }
 }
 
 // Workgroups of each instance resident per CU, from the runtime (0 where it
-// cannot say, e.g. with no device), cached per column-tile shape.
+// cannot say, e.g. with no device), cached per shape.
 template <typename K>
-static int blocks_per_cu(K kern)
+static int blocks_per_cu(K kern, int threads)
 {
     int n = 0;
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, 512, 0) == hipSuccess ? n : 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, threads, 0) == hipSuccess ? n : 0;
 }
 
-template <int SA, int F, int RT>
+template <int NS, int SA, int F, int RT, int W>
 static int zt_occ()
 {
-    if constexpr (SA * (3 * F + RT) > 36) return 0;
-    else return blocks_per_cu(k_zt_i8<SA, F, RT>);
+    if constexpr (!zt_fits(SA, NS * F + RT, W)) return 0;
+    else return blocks_per_cu(k_zt_i8<NS, SA, F, RT, W>, 64 * W);
 }
 
-template <int ST, int F, int RT>
+template <int NS, int ST, int F, int RT, int W>
 static int za_occ()
 {
-    if constexpr (ST * (3 * F + RT) > 18) return 0;
-    else return blocks_per_cu(k_za_i8<ST, F, RT, false>);
+    if constexpr (!za_fits(ST, NS * F + RT, W)) return 0;
+    else return blocks_per_cu(k_za_i8<NS, ST, F, RT, false, W>, 64 * W);
 }
 
+// Tile counts per wave, largest first: Z^T T sample tiles, Z Q SNP tiles.
+constexpr int kZtTiles[3] = {8, 4, 2};
+constexpr int kZaTiles[3] = {4, 2, 1};
+
 struct I8Occ {
-    int zt4, zt2, za2, za1;
+    int zt[2][3], za[2][3];  // [waves 8, 4][tiles in kZtTiles / kZaTiles order]
 };
 
-static I8Occ i8_occ(int k)
+static I8Occ i8_occ(int k, int ns)
 {
     static std::mutex mu;
     static int dev = -1;
-    static I8Occ cache[kMaxKp + 1];
-    static bool have[kMaxKp + 1];
+    static I8Occ cache[2][kMaxKp + 1];
+    static bool have[2][kMaxKp + 1];
+    I8Occ none{};
     int d = 0;
-    if (hipGetDevice(&d) != hipSuccess) return I8Occ{0, 0, 0, 0};
+    if (hipGetDevice(&d) != hipSuccess) return none;
     std::lock_guard<std::mutex> lk(mu);
     if (d != dev) {
-        for (bool &h : have) h = false;
+        for (auto &hv : have)
+            for (bool &h : hv) h = false;
         dev = d;
     }
-    if (k < 1 || k > kMaxKp) return I8Occ{0, 0, 0, 0};
-    if (!have[k]) {
-        const ColTiles ct = col_tiles(k);
-        I8Occ o{0, 0, 0, 0};
-#define X(F_, RT_)                                                                          \
-    if (ct.F == F_ && ct.RT == RT_)                                                          \
-        o = I8Occ{zt_occ<4, F_, RT_>(), zt_occ<2, F_, RT_>(), za_occ<2, F_, RT_>(),           \
-                  za_occ<1, F_, RT_>()};
+    if (k < 1 || k > kMaxKp || (ns != 2 && ns != 3)) return none;
+    const int ni = ns - 2;
+    if (!have[ni][k]) {
+        const ColTiles ct = col_tiles(k, ns);
+        I8Occ o{};
+#define OCC_W(NS_, F_, RT_, W_, wi)                                                         \
+    o.zt[wi][0] = zt_occ<NS_, 8, F_, RT_, W_>();                                             \
+    o.zt[wi][1] = zt_occ<NS_, 4, F_, RT_, W_>();                                             \
+    o.zt[wi][2] = zt_occ<NS_, 2, F_, RT_, W_>();                                             \
+    o.za[wi][0] = za_occ<NS_, 4, F_, RT_, W_>();                                             \
+    o.za[wi][1] = za_occ<NS_, 2, F_, RT_, W_>();                                             \
+    o.za[wi][2] = za_occ<NS_, 1, F_, RT_, W_>();
+#define X(NS_, F_, RT_)                                                                     \
+    if (ct.NS == NS_ && ct.F == F_ && ct.RT == RT_) {                                        \
+        OCC_W(NS_, F_, RT_, 8, 0)                                                            \
+        OCC_W(NS_, F_, RT_, 4, 1)                                                            \
+    }
         DPCA_I8_SHAPES(X)
 #undef X
-        // an unknown answer is not cached
-        if (o.zt4 <= 0 || o.zt2 <= 0 || o.za2 <= 0 || o.za1 <= 0) return o;
-        cache[k] = o;
-        have[k] = true;
+#undef OCC_W
+        // an unknown answer (no device) is not cached
+        bool known = false;
+        for (int w = 0; w < 2; w++)
+            for (int i = 0; i < 3; i++) known |= o.zt[w][i] > 0 || o.za[w][i] > 0;
+        if (!known) return o;
+        cache[ni][k] = o;
+        have[ni][k] = true;
     }
-    return cache[k];
+    return cache[ni][k];
 }
 
-// Two tiles per wave reuse each operand word from LDS over twice the rows or
-// samples of one tile; one tile per wave (two sample tiles for Z^T T) is
-// chosen where it doubles the workgroups resident per CU instead.
-int zt_i8_tiles(int k, int64_t ldb)
+static int wave_index(int waves) { return waves == 4 ? 1 : 0; }
+
+bool zt_i8_valid(int k, int ns, int sa_tiles, int64_t ldb, int waves)
 {
-    if (col_tiles(k).vt() > 9 || ldb % 128) return 2;
-    const I8Occ o = i8_occ(k);
-    return (o.zt4 > 0 && o.zt2 >= 2 * o.zt4) ? 2 : 4;
+    if ((ns != 2 && ns != 3) || k < 1 || k > kMaxKp || (waves != 8 && waves != 4)) return false;
+    const int vt = col_tiles(k, ns).vt();
+    return (sa_tiles == 8 || sa_tiles == 4 || sa_tiles == 2) && zt_fits(sa_tiles, vt, waves) &&
+           ldb % zt_tile_bytes(sa_tiles, waves) == 0;
 }
 
-int za_i8_tiles(int k)
+bool za_i8_valid(int k, int ns, int st_tiles, int waves)
 {
-    if (col_tiles(k).vt() > 9) return 1;
-    const I8Occ o = i8_occ(k);
-    return (o.za2 > 0 && o.za1 >= 2 * o.za2) ? 1 : 2;
+    if ((ns != 2 && ns != 3) || k < 1 || k > kMaxKp || (waves != 8 && waves != 4)) return false;
+    return (st_tiles == 4 || st_tiles == 2 || st_tiles == 1) &&
+           za_fits(st_tiles, col_tiles(k, ns).vt(), waves);
 }
 
-int zt_i8_blocks_per_cu(int k, int sa_tiles)
+// The largest tile count that fits, stepping down while the next smaller
+// form doubles the workgroups resident per CU (more tiles per wave reuse
+// each LDS operand word more often; more resident waves hide latency).  For
+// Z^T T the step down pays only with at most 4 column tiles (measured: k 20
+// with 3 slices 1.52 -> 1.23 ms at 20k x 500k; k 40 with 2 slices, 5 tiles,
+// 13.2 ms with 4 sample tiles against 14.1 ms with 2 at 100k x 1M).
+int zt_i8_tiles(int k, int ns, int64_t ldb, int waves)
 {
-    const I8Occ o = i8_occ(k);
-    return sa_tiles == 4 ? o.zt4 : sa_tiles == 2 ? o.zt2 : 0;
+    const I8Occ o = i8_occ(k, ns);
+    const int wi = wave_index(waves);
+    const bool narrow = col_tiles(k, ns).vt() <= 4;
+    int pick = -1;
+    for (int i = 0; i < 3; i++) {
+        if (!zt_i8_valid(k, ns, kZtTiles[i], ldb, waves)) continue;
+        if (pick < 0 || (narrow && o.zt[wi][pick] > 0 && o.zt[wi][i] >= 2 * o.zt[wi][pick]))
+            pick = i;
+        else
+            break;
+    }
+    return pick < 0 ? 2 : kZtTiles[pick];
 }
 
-// ldp must be a multiple of the tile's byte width (SA*32), rows and
-// rows_per_split multiples of 128, rows_per_split at most 2^22 (int32 sums).
+int za_i8_tiles(int k, int ns, int waves)
+{
+    const I8Occ o = i8_occ(k, ns);
+    const int wi = wave_index(waves);
+    int pick = -1;
+    for (int i = 0; i < 3; i++) {
+        if (!za_i8_valid(k, ns, kZaTiles[i], waves)) continue;
+        if (pick < 0 || (o.za[wi][pick] > 0 && o.za[wi][i] >= 2 * o.za[wi][pick])) pick = i;
+        else break;
+    }
+    return pick < 0 ? 1 : kZaTiles[pick];
+}
+
+int zt_i8_blocks_per_cu(int k, int ns, int sa_tiles, int waves)
+{
+    const I8Occ o = i8_occ(k, ns);
+    for (int i = 0; i < 3; i++)
+        if (kZtTiles[i] == sa_tiles) return o.zt[wave_index(waves)][i];
+    return 0;
+}
+
+int64_t zt_i8_tile_bytes(int sa_tiles, int waves) { return zt_tile_bytes(sa_tiles, waves); }
+
+// ldp a multiple of the tile's byte width, rows and rows_per_split
+// multiples of 128, rows_per_split at most 2^22 (int32 sums).
 hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const void *UW8, const unsigned *smax, int k, int sa_tiles,
-                        int64_t rows_per_split, int nsplit, float *part)
+                        const void *UW8, const unsigned *smax, int k, int ns, int sa_tiles,
+                        int waves, int64_t rows_per_split, int nsplit, float *part)
 {
-    const ColTiles ct = col_tiles(k);
-    const int64_t tb = (int64_t)sa_tiles * 32;
-    if (ldp % tb || rows % 128 || rows_per_split % 128 || rows_per_split > (1 << 22))
+    if (!zt_i8_valid(k, ns, sa_tiles, ldp, waves) || rows % 128 || rows_per_split % 128 ||
+        rows_per_split > (1 << 22))
         return hipErrorInvalidValue;
-    dim3 grid((unsigned)(ldp / tb), (unsigned)nsplit);
+    const ColTiles ct = col_tiles(k, ns);
+    dim3 grid((unsigned)(ldp / zt_tile_bytes(sa_tiles, waves)), (unsigned)nsplit);
     const int64_t Mpad = 4 * ldp;
     const u32x4 *uw = reinterpret_cast<const u32x4 *>(UW8);
-#define X(F_, RT_)                                                                          \
-    if (ct.F == F_ && ct.RT == RT_) {                                                        \
-        if (sa_tiles == 4)                                                                   \
-            return zt_go<4, F_, RT_>(st, grid, packed, ldp, uw, smax, rows, rows_per_split,  \
-                                     Mpad, ct.R, part);                                      \
-        if (sa_tiles == 2)                                                                   \
-            return zt_go<2, F_, RT_>(st, grid, packed, ldp, uw, smax, rows, rows_per_split,  \
-                                     Mpad, ct.R, part);                                      \
+#define ZT_T(NS_, SA_, F_, RT_, W_)                                                         \
+    if (sa_tiles == SA_ && waves == W_)                                                      \
+        return zt_go<NS_, SA_, F_, RT_, W_>(st, grid, packed, ldp, uw, smax, rows,           \
+                                            rows_per_split, Mpad, ct.R, part);
+#define X(NS_, F_, RT_)                                                                     \
+    if (ct.NS == NS_ && ct.F == F_ && ct.RT == RT_) {                                        \
+        ZT_T(NS_, 8, F_, RT_, 8) ZT_T(NS_, 4, F_, RT_, 8) ZT_T(NS_, 2, F_, RT_, 8)           \
+        ZT_T(NS_, 8, F_, RT_, 4) ZT_T(NS_, 4, F_, RT_, 4) ZT_T(NS_, 2, F_, RT_, 4)           \
         return hipErrorInvalidValue;                                                         \
     }
     DPCA_I8_SHAPES(X)
 #undef X
+#undef ZT_T
     return hipErrorInvalidValue;
 }
 
-// rows a multiple of 8*16*st_tiles, seg_bytes of 64, segments of at most
-// 65536 samples (int32 sums).
+// rows a multiple of waves*16*st_tiles, seg_bytes of 128, segments of at
+// most 65536 samples (int32 sums).
 hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
                         const void *Q8, const unsigned *qmax, const double *mu,
-                        const double *dsc, int k, int st_tiles, int64_t seg_bytes, int nseg,
-                        float *Tpart)
+                        const double *dsc, int k, int ns, int st_tiles, int waves,
+                        int64_t seg_bytes, int nseg, float *Tpart)
 {
-    const ColTiles ct = col_tiles(k);
-    const int64_t per = 8 * 16 * (int64_t)st_tiles;
+    const int64_t per = (int64_t)waves * 16 * st_tiles;
     // |A| <= 126 per sample: a segment of <= 2^16 samples stays exact in int32
-    if (rows % per || seg_bytes % 128 || ldp % 128 || seg_bytes > 16384)
+    if (!za_i8_valid(k, ns, st_tiles, waves) || rows % per || seg_bytes % 128 || ldp % 128 ||
+        seg_bytes > 16384)
         return hipErrorInvalidValue;
+    const ColTiles ct = col_tiles(k, ns);
     dim3 grid((unsigned)(rows / per), (unsigned)nseg);
     const u32x4 *q = reinterpret_cast<const u32x4 *>(Q8);
-    // the last remainder tile holds only slice 2 when the slices 0 and 1 of
-    // the R remainder columns fit the tiles before it
-    const bool skr = ct.RT > 0 && 2 * ct.R <= 16 * (ct.RT - 1);
-#define X(F_, RT_)                                                                          \
-    if (ct.F == F_ && ct.RT == RT_) {                                                        \
-        if (st_tiles == 2 && skr)                                                            \
-            return za_go<2, F_, RT_, true>(st, grid, packed, ldp, q, qmax, mu, dsc,          \
-                                           seg_bytes, rows, ct.R, Tpart);                    \
-        if (st_tiles == 2)                                                                   \
-            return za_go<2, F_, RT_, false>(st, grid, packed, ldp, q, qmax, mu, dsc,         \
-                                            seg_bytes, rows, ct.R, Tpart);                   \
-        if (st_tiles == 1 && skr)                                                            \
-            return za_go<1, F_, RT_, true>(st, grid, packed, ldp, q, qmax, mu, dsc,          \
-                                           seg_bytes, rows, ct.R, Tpart);                    \
-        if (st_tiles == 1)                                                                   \
-            return za_go<1, F_, RT_, false>(st, grid, packed, ldp, q, qmax, mu, dsc,         \
-                                            seg_bytes, rows, ct.R, Tpart);                   \
+    // the last remainder tile holds only the last slice when the other
+    // slices of the R remainder columns fit the tiles before it
+    const bool skr = ct.RT > 0 && (ns - 1) * ct.R <= 16 * (ct.RT - 1);
+#define ZA_T(NS_, ST_, F_, RT_, W_)                                                         \
+    if (st_tiles == ST_ && waves == W_) {                                                    \
+        if (skr)                                                                             \
+            return za_go<NS_, ST_, F_, RT_, true, W_>(st, grid, packed, ldp, q, qmax, mu,    \
+                                                      dsc, seg_bytes, rows, ct.R, Tpart);    \
+        return za_go<NS_, ST_, F_, RT_, false, W_>(st, grid, packed, ldp, q, qmax, mu, dsc,  \
+                                                   seg_bytes, rows, ct.R, Tpart);            \
+    }
+#define X(NS_, F_, RT_)                                                                     \
+    if (ct.NS == NS_ && ct.F == F_ && ct.RT == RT_) {                                        \
+        ZA_T(NS_, 4, F_, RT_, 8) ZA_T(NS_, 2, F_, RT_, 8) ZA_T(NS_, 1, F_, RT_, 8)           \
+        ZA_T(NS_, 4, F_, RT_, 4) ZA_T(NS_, 2, F_, RT_, 4) ZA_T(NS_, 1, F_, RT_, 4)           \
         return hipErrorInvalidValue;                                                         \
     }
     DPCA_I8_SHAPES(X)
 #undef X
+#undef ZA_T
     return hipErrorInvalidValue;
 }
 

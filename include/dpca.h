@@ -56,6 +56,10 @@ typedef struct dpca_params {
                                DPCA_HOST_FILE (dpca_open only: every pass
                                reads the .bed file again, as the reference's
                                MPI_File_iread_at loop, src/methods.cpp:795-811) */
+    int slices;             /* mixed form: int8 slices of the FP64 operand of
+                               each product, 2 or 3 (0 = the default, 3);
+                               2 issues a third fewer MFMAs at ~100x the
+                               pass error of 3                               */
 } dpca_params;
 
 enum { DPCA_HOST_PINNED = 0, DPCA_HOST_PAGEABLE = 1, DPCA_HOST_FILE = 2 };

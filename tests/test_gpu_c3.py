@@ -4,9 +4,10 @@ HBM (8 Balding-Nichols populations, Fst 0.005, 20% Dirichlet-admixed samples,
 from pageable host memory through a 2 GB device cap (10 populations, Fst
 0.005, 1% missing, seed 3; nsv 20, nrhs 40, power 4, bsize 65536).
 
-* The full 100k x 1M PCA in both forms: the mixed form stops at the FP64
-  form's iteration with every eigenvalue within 1e-5 (relative) and the 7
-  population PCs within 1e-4 rad.
+* The full 100k x 1M PCA in both forms: the mixed form (3 and 2 int8
+  slices) stops at the FP64 form's iteration with every eigenvalue within
+  1e-5 (relative) and the 7 population PCs within 1e-4 rad (3 slices: every
+  top-j subspace).
 * A 16,384-SNP slice of the same generator at M = 100,000 -- past M = 46,341,
   where the reference's Omega seed jj*M + kk wraps (src/distpca.cpp:389-394)
   -- against the oracle: per-SNP statistics bit-exact, the FP64 pass within
@@ -39,23 +40,31 @@ def _angles(U, V, js):
 
 
 def test_c3_full_mixed_vs_fp64():
+    # both slice counts of the mixed form against the FP64 form.  The 7
+    # population PCs' eigenvalues lie within 3e-4 of each other, so single
+    # vectors inside that cluster are not converged after 9 iterations in
+    # either form; the population subspace (top 7) and every eigenvalue are.
     M, N = 100000, 1000000
     spec = D.gen_spec(M, N, **C3)
     base = dict(nsv=20, nrhs=40, power=6)
     out = {}
-    for name, prec in (("mixed", D.MIXED), ("fp64", D.FP64)):
+    for name, prec, sl in (("fp64", D.FP64, 0), ("mixed3", D.MIXED, 3), ("mixed2", D.MIXED, 2)):
         _banner(f"c3 full PCA {M} x {N}, {name}")
-        with D.PCA.synthetic(spec, D.Params(precision=prec, **base)) as pca:
+        with D.PCA.synthetic(spec, D.Params(precision=prec, slices=sl, **base)) as pca:
             out[name] = pca.run()
-    (sm, Um, rm), (sf, Uf, rf) = out["mixed"], out["fp64"]
-    rel = np.abs(sm ** 2 - sf ** 2) / sf ** 2
-    ang = _angles(Um, Uf, (1, 4, 7, 20))
-    _banner(f"c3: its mixed {rm['total_its']} fp64 {rf['total_its']}; eigenvalue rel err "
-            f"max {rel.max():.2e}; angles {ang}")
-    assert rm["total_its"] == rf["total_its"]
-    assert rm["passes"] == rf["passes"] == 7 * rf["total_its"]
-    assert rel.max() <= 1e-5
-    assert ang[7] <= 1e-4
+    sf, Uf, rf = out["fp64"]
+    for name in ("mixed3", "mixed2"):
+        sm, Um, rm = out[name]
+        rel = np.abs(sm ** 2 - sf ** 2) / sf ** 2
+        ang = _angles(Um, Uf, (1, 4, 7, 20))
+        _banner(f"c3 {name}: its {rm['total_its']} (fp64 {rf['total_its']}); eigenvalue rel err "
+                f"max {rel.max():.2e}; angles {ang}")
+        assert rm["total_its"] == rf["total_its"]
+        assert rm["passes"] == rf["passes"] == 7 * rf["total_its"]
+        assert rel.max() <= 1e-5
+        assert ang[7] <= 1e-4
+        if name == "mixed3":  # three slices: every top-j subspace within 1e-4 rad
+            assert max(ang.values()) <= 1e-4
 
 
 @pytest.fixture(scope="module")
