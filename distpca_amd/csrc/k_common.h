@@ -21,6 +21,21 @@ inline unsigned ew_blocks(int64_t n)
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+// Tile (x, y) of a gx x gy tiling run as a 1-D grid of gx * gy workgroups.
+// The dispatcher hands workgroup L to XCD L % 8; the tiles are ordered so
+// that each XCD works through consecutive rows y (a product's operand for
+// row y -- a sample segment of Q slices, an SNP split of U/W slices -- is
+// then fetched into that XCD's L2 once rather than into all eight).  The
+// last T mod 8 workgroups keep their own index.
+__device__ __forceinline__ void xcd_tile(int gx, int &x, int &y)
+{
+    const int T = (int)gridDim.x, L = (int)blockIdx.x;
+    const int T8 = T / 8 * 8;
+    const int V = L < T8 ? (L % 8) * (T8 / 8) + L / 8 : L;
+    x = V % gx;
+    y = V / gx;
+}
+
 // 64-wide wavefront sum.
 __device__ __forceinline__ double wave_sum(double v)
 {

@@ -318,8 +318,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     __shared__ u32x4 uw[2][NUWT];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
-    const int64_t mb = (int64_t)blockIdx.x * TB;
-    const int64_t sbeg = (int64_t)blockIdx.y * rows_per_split;
+    int bx, by;  // sample tile, SNP split
+    xcd_tile((int)(ldp / TB), bx, by);
+    const int64_t mb = (int64_t)bx * TB;
+    const int64_t sbeg = (int64_t)by * rows_per_split;
     int64_t send = sbeg + rows_per_split;
     if (send > rows) send = rows;
     const uint32_t HT = 0x02010000u, MT = 0x01010001u;  // h, m by 2-bit code
@@ -455,11 +457,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
 #undef DPCA_ZT8_STORE
 #undef DPCA_ZT8_LOADU
 #undef DPCA_ZT8_LOADP
-    float *out = part + ((int64_t)blockIdx.y * Mpad + 4 * mb + wave * SA * 16) * KP;
+    float *out = part + ((int64_t)by * Mpad + 4 * mb + wave * SA * 16) * KP;
 #pragma unroll
     for (int oc = 0; oc < CT; oc++) {
         const int col = oc * 16 + r;
-        const double s = slice_scale(__uint_as_float(smax[blockIdx.y * KP + col]));
+        const double s = slice_scale(__uint_as_float(smax[by * KP + col]));
 #pragma unroll
         for (int a = 0; a < SA; a++)
 #pragma unroll
@@ -554,8 +556,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
     __shared__ u32x4 qs[2][NQT];               // double buffer, one barrier per chunk
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
-    const int64_t base = (int64_t)blockIdx.x * (WAVES * ST * 16) + wave * (ST * 16);
-    const int64_t b0 = (int64_t)blockIdx.y * seg_bytes;
+    int bx, by;  // SNP row block, sample segment
+    xcd_tile((int)(Npad / (WAVES * ST * 16)), bx, by);
+    const int64_t base = (int64_t)bx * (WAVES * ST * 16) + wave * (ST * 16);
+    const int64_t b0 = (int64_t)by * seg_bytes;
     int64_t b1 = b0 + seg_bytes;
     if (b1 > ldp) b1 = ldp;
     const uint32_t MT = 0x01010001u;  // m by 2-bit code
@@ -706,11 +710,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
     }
 #undef DPCA_ZA8_STORE
 #undef DPCA_ZA8_LOAD
-    float *out = Tpart + (int64_t)blockIdx.y * Npad * KP;
+    float *out = Tpart + (int64_t)by * Npad * KP;
 #pragma unroll
     for (int oc = 0; oc < CT; oc++) {
         const int col = oc * 16 + r;
-        const double sc = slice_scale(__uint_as_float(qmax[blockIdx.y * KP + col]));
+        const double sc = slice_scale(__uint_as_float(qmax[by * KP + col]));
 #pragma unroll
         for (int s = 0; s < ST; s++)
 #pragma unroll
@@ -927,7 +931,8 @@ hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int
         rows_per_split > (1 << 22))
         return hipErrorInvalidValue;
     const ColTiles ct = col_tiles(k, ns);
-    dim3 grid((unsigned)(ldp / zt_tile_bytes(sa_tiles, waves)), (unsigned)nsplit);
+    // 1-D grid of tiles x splits, ordered per XCD inside the kernel
+    dim3 grid((unsigned)(ldp / zt_tile_bytes(sa_tiles, waves) * nsplit));
     const int64_t Mpad = 4 * ldp;
     const u32x4 *uw = reinterpret_cast<const u32x4 *>(UW8);
 #define ZT_T(NS_, SA_, F_, RT_, W_)                                                         \
@@ -959,7 +964,8 @@ hipError_t launch_za_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int
         seg_bytes > 16384)
         return hipErrorInvalidValue;
     const ColTiles ct = col_tiles(k, ns);
-    dim3 grid((unsigned)(rows / per), (unsigned)nseg);
+    // 1-D grid of row blocks x segments, ordered per XCD inside the kernel
+    dim3 grid((unsigned)(rows / per * nseg));
     const u32x4 *q = reinterpret_cast<const u32x4 *>(Q8);
     // the last remainder tile holds only the last slice when the other
     // slices of the R remainder columns fit the tiles before it

@@ -284,7 +284,10 @@ class OOCPass:
     """One pass A A^T Q over the SNP blocks (src/methods.cpp:786-936): decode
     block, T = Z_b Q, ARHS += Z_b^T T."""
 
-    def __init__(self, packed: np.ndarray, M: int, N: int, bsize: int):
+    def __init__(self, packed: np.ndarray, M: int, N: int, bsize: int, progress=None):
+        # progress(j, nblk), when given, is called after each block (long
+        # CPU runs report their progress through it)
+        self.progress = progress
         self.packed = packed.reshape(-1)
         self.M, self.N, self.b = M, N, bsize
         self.np = (M + 3) // 4
@@ -314,6 +317,8 @@ class OOCPass:
             self.t_load += t1 - t0
             self.t_mm_a += t2 - t1
             self.t_mm_at += t3 - t2
+            if self.progress:
+                self.progress(j, nblk)
         self.count += 1
         return ARHS
 

@@ -131,3 +131,47 @@ def test_server_rejects_job_beyond_budget():
         srv.submit_job(D.gen_spec(20000, 200000, npop=2), D.Params(nsv=2, nrhs=4))
     assert srv.counters()["jobs_rejected"] == 1
     srv.close()
+
+
+def test_serving_at_full_loading_set_size():
+    # the serving workload's own shapes: a resident 1,000,000-SNP, nsv-20
+    # loading set (5 populations, Fst 0.01, 20,000 training samples, mixed
+    # form), projection requests from 1 to 4,096 new samples and a cohort job
+    # of 2,000 x 100,000 running beside them.  Each request's scores are
+    # bitwise the same alone and in the mix; training samples project onto
+    # their own rows of U on the converged population PCs.
+    Nl, Ml = 1000000, 20000
+    print(f"[gpu] serving at {Ml} x {Nl}, nsv 20, mixed", flush=True)
+    spec = D.gen_spec(Ml, Nl, npop=5, fst=0.01, missing=0.01, seed=1)
+    p = D.Params(nsv=20, nrhs=40, power=2, crit=1, tol=1e-6, miter=30, precision=D.MIXED)
+    with D.PCA.synthetic(spec, p) as pca:
+        sigma, rhs, rep = pca.run()
+        model = D.Model(pca, rhs, sigma, 20)
+        train = np.ascontiguousarray(pca.packed()[:, :16])       # samples 0..63
+    own = model.project(train, 64)
+    # 5 populations: PCs 1-4 carry the structure and are converged
+    scale = np.abs(rhs[:64, :4]).max()
+    assert np.abs(own[:, :4] - rhs[:64, :4]).max() <= 1e-4 * scale
+    new = D.PCA.synthetic(D.gen_spec(4096, Nl, npop=5, fst=0.01, missing=0.02, seed=12),
+                          D.Params(nsv=2, nrhs=4))
+    pool = new.packed()
+    new.close()
+    srv = D.Server(model, hbm_budget=int(100e9), batch_capacity=16384, workers=2)
+    sizes = (1, 3, 37, 300, 1500, 4096)
+    reqs = {n: np.ascontiguousarray(pool[:, :(n + 3) // 4]) for n in sizes}
+    alone = {}
+    for n, pay in reqs.items():
+        alone[n], _ = srv.wait(srv.submit_project(pay, n))
+    jspec = D.gen_spec(2000, 100000, npop=5, fst=0.01, missing=0.01, seed=1005)
+    jpar = D.Params(nsv=10, nrhs=20, precision=D.MIXED)
+    sig_alone, _ = srv.wait(srv.submit_job(jspec, jpar))
+    tj = srv.submit_job(jspec, jpar)
+    ts = {n: srv.submit_project(pay, n) for n, pay in reqs.items()}
+    for n, t in ts.items():
+        sc, info = srv.wait(t)
+        assert np.array_equal(sc, alone[n]), n
+    sig_mix, jinfo = srv.wait(tj)
+    assert np.array_equal(sig_mix, sig_alone)
+    print(f"[gpu] serving: job sigma[:3] {sig_mix[:3]}, footprint {jinfo['footprint']}", flush=True)
+    srv.close()
+    model.close()
