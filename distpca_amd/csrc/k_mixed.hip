@@ -339,14 +339,21 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     // current one is multiplied, then stored into the other LDS buffer
     uint32_t rpk[NPK][4];
     u32x4 ruw[NUW];
+    // each thread's byte offset within a chunk (32 bits: < 32 rows of ldp
+    // bytes); the chunk's own offset is uniform, so the address arithmetic of
+    // a chunk's loads is scalar work plus one add
+    uint32_t poff[NPK];
+#pragma unroll
+    for (int u = 0; u < NPK; u++) {
+        const int it = tid + NT * u;
+        poff[u] = (uint32_t)((4 * (it / (TB / 4))) * ldp + mb + 4 * (it % (TB / 4)));
+    }
 #define DPCA_ZT8_LOADP(c0, dst)                                                             \
     do {                                                                                    \
-        _Pragma("unroll") for (int u = 0; u < NPK; u++) {                                  \
-            const int it_ = tid + NT * u;                                                   \
-            const int sg_ = it_ / (TB / 4), bq_ = it_ % (TB / 4);                           \
-            const uint8_t *p_ = packed + ((c0) + 4 * sg_) * ldp + mb + 4 * bq_;             \
-            _Pragma("unroll") for (int j = 0; j < 4; j++) dst[u][j] =                      \
-                *reinterpret_cast<const uint32_t *>(p_ + j * ldp);                          \
+        _Pragma("unroll") for (int j = 0; j < 4; j++) {                                    \
+            const uint8_t *b_ = packed + ((c0) + j) * ldp;                                  \
+            _Pragma("unroll") for (int u = 0; u < NPK; u++) dst[u][j] =                    \
+                *reinterpret_cast<const uint32_t *>(b_ + poff[u]);                          \
         }                                                                                   \
     } while (0)
 #define DPCA_ZT8_LOADU(c0)                                                                  \
