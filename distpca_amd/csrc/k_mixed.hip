@@ -115,6 +115,27 @@ __device__ __forceinline__ i32x4 mfma_i8(const u32x4 &a, const u32x4 &b, const i
                                                  __builtin_bit_cast(i32x4, b), c, 0, 0, 0);
 }
 
+// Slice operands reach LDS without passing through registers: one
+// global_load_lds_dwordx4 copies 16 bytes per lane from src into LDS at
+// lds + 16 lane, lds being the wave's (uniform) LDS byte address, set in M0.
+// The compiler does not see this write, so (a) nothing in the kernel stores
+// to the destination buffer through ordinary code, (b) the buffer is read
+// only after the issuing waves have waited for their copies (lds_dma_wait)
+// and passed a barrier, and (c) it is issued only into a buffer whose
+// previous readers have all passed a barrier since.  The instruction has no
+// register result; its address operand is read at issue.  M0 is reserved by
+// the compiler and not otherwise used by the kernels that call this (no
+// message, GWS, indexed-move or compiler-issued LDS-DMA instructions), so it
+// is written here without being declared.
+__device__ __forceinline__ void lds_dma16(uint32_t lds, const void *src)
+{
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" : : "s"(lds), "v"(src) : "memory");
+}
+// all of this wave's global loads, the LDS copies among them, have landed
+__device__ __forceinline__ void lds_dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// LDS byte address of a __shared__ object (the low half of its generic address)
+__device__ __forceinline__ uint32_t lds_addr(const void *p) { return (uint32_t)(uintptr_t)p; }
+
 // power-of-two scale s with max / s < 127 (s = 1 for an all-zero column)
 __device__ __forceinline__ double slice_scale(float mx)
 {
@@ -335,10 +356,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     constexpr int NPK = (SC / 4) * (TB / 4) / NT;   // 4x4 byte blocks per thread
     constexpr int NUW = (NUWT + NT - 1) / NT;       // UW8 words per thread
     static_assert((SC / 4) * (TB / 4) % NT == 0, "tile");
-    // software pipeline: the next chunk is loaded into registers while the
-    // current one is multiplied, then stored into the other LDS buffer
+    // software pipeline: while the current chunk is multiplied, the next
+    // chunk's packed codes are loaded into registers (stored into the other
+    // LDS buffer, transposed, at the end of the chunk) and its UW8 words are
+    // copied straight into the other buffer (lds_dma16)
     uint32_t rpk[NPK][4];
-    u32x4 ruw[NUW];
     // each thread's byte offset within a chunk (32 bits: < 32 rows of ldp
     // bytes); the chunk's own offset is uniform, so the address arithmetic of
     // a chunk's loads is scalar work plus one add
@@ -356,12 +378,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
                 *reinterpret_cast<const uint32_t *>(b_ + poff[u]);                          \
         }                                                                                   \
     } while (0)
-#define DPCA_ZT8_LOADU(c0)                                                                  \
+    // lane l of wave w copies word NT u + 64 w + l of the chunk's UW8 words
+    // (NUWT is a multiple of 64, so each wave's test is uniform)
+    const uint32_t uw_lds = lds_addr(&uw[0][0]);
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+#define DPCA_ZT8_LOADU(c0, buf)                                                             \
     do {                                                                                    \
         const u32x4 *q_ = UW8 + ((c0) / 8) * VT * 16;                                       \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
-            const int it_ = tid + NT * u;                                                   \
-            if (it_ < NUWT) ruw[u] = q_[it_];                                               \
+            const int w_ = NT * u + 64 * wave_u;                                            \
+            if (w_ < NUWT)                                                                  \
+                lds_dma16(uw_lds + 16u * (uint32_t)((buf) * NUWT + w_), q_ + w_ + lane);    \
         }                                                                                   \
     } while (0)
     // 4x4 byte transpose of each block into [byte column][SNP group]
@@ -381,17 +408,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
             pk[buf][4 * bq_ + 2][col_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x05040100u);  \
             pk[buf][4 * bq_ + 3][col_] = __builtin_amdgcn_perm(hi23_, hi01_, 0x07060302u);  \
         }                                                                                   \
-        _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
-            const int it_ = tid + NT * u;                                                   \
-            if (it_ < NUWT) uw[buf][it_] = ruw[u];                                          \
-        }                                                                                   \
     } while (0)
+    static_assert(NUWT % 64 == 0, "whole waves of UW8 words");
     {
         const int64_t c0 = sbeg < send ? sbeg : 0;
         DPCA_ZT8_LOADP(c0, rpk);
-        DPCA_ZT8_LOADU(c0);
+        DPCA_ZT8_LOADU(c0, 0);
         DPCA_ZT8_STORE(0);
     }
+    lds_dma_wait();
     __syncthreads();
 
     int cur = 0;
@@ -400,7 +425,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
             // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
             DPCA_ZT8_LOADP(nx, rpk);
-            DPCA_ZT8_LOADU(nx);
+            DPCA_ZT8_LOADU(nx, cur ^ 1);  // last read before the previous barrier
         }
         // The chunk's B tiles (K step t, tile v) form one sequence i = t VT + v
         // read from LDS ZB tiles ahead of their MFMAs, and the packed words
@@ -459,6 +484,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
             __builtin_amdgcn_sched_barrier(0x4);
         }
         DPCA_ZT8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
+        lds_dma_wait();
         __syncthreads();
     }
 #undef DPCA_ZT8_STORE
@@ -596,31 +622,32 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
             accm[s][v] = i32x4{0, 0, 0, 0};
         }
 
-    // software pipeline: the next chunk's Q8 words and packed bytes are
-    // loaded into registers while the current chunk is multiplied; the Q8
-    // words then go to the other LDS buffer
-    u32x4 rq[NQ], rp[ST][NW];
-#define DPCA_ZA8_LOAD(c0)                                                                   \
+    // software pipeline: while the current chunk is multiplied, the next
+    // chunk's packed bytes are loaded into registers and its Q8 words copied
+    // straight into the other LDS buffer (lds_dma16; lane l of wave w copies
+    // word NT u + 64 w + l, NQT being a multiple of 64)
+    static_assert(NQT % 64 == 0, "whole waves of Q8 words");
+    const uint32_t qs_lds = lds_addr(&qs[0][0]);
+    const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+    u32x4 rp[ST][NW];
+#define DPCA_ZA8_LOADQ(c0, buf)                                                             \
     do {                                                                                    \
         const u32x4 *src_ = Q8 + ((c0) / 4) * VT * 16;                                      \
         _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
-            const int it_ = tid + NT * u;                                                   \
-            if (it_ < NQT) rq[u] = src_[it_];                                               \
+            const int w_ = NT * u + 64 * wave_u;                                            \
+            if (w_ < NQT)                                                                   \
+                lds_dma16(qs_lds + 16u * (uint32_t)((buf) * NQT + w_), src_ + w_ + lane);   \
         }                                                                                   \
+    } while (0)
+#define DPCA_ZA8_LOADP(c0)                                                                  \
+    do {                                                                                    \
         _Pragma("unroll") for (int s = 0; s < ST; s++)                                     \
             _Pragma("unroll") for (int n = 0; n < NW; n++) rp[s][n] =                      \
                 *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + (c0) + \
                                                  (CB / 4) * g + 16 * n);                    \
     } while (0)
-#define DPCA_ZA8_STORE(buf)                                                                 \
-    do {                                                                                    \
-        _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
-            const int it_ = tid + NT * u;                                                   \
-            if (it_ < NQT) qs[buf][it_] = rq[u];                                            \
-        }                                                                                   \
-    } while (0)
-    DPCA_ZA8_LOAD(b0 < b1 ? b0 : 0);
-    DPCA_ZA8_STORE(0);
+    DPCA_ZA8_LOADQ(b0 < b1 ? b0 : 0, 0);
+    DPCA_ZA8_LOADP(b0 < b1 ? b0 : 0);
     // where the registers allow, the packed codes (HBM) run two chunks
     // ahead and the Q8 words (L2) one
     constexpr bool PF2 = ST * VT < 16;
@@ -634,6 +661,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
                 rp2[s][n] = *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + c2 +
                                                              (CB / 4) * g + 16 * n);
     }
+    lds_dma_wait();
     __syncthreads();
 
     int cur = 0;
@@ -646,14 +674,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
         {
             // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (cb + CB < b1) ? cb + CB : cb;
+            DPCA_ZA8_LOADQ(nx, cur ^ 1);  // last read before the previous barrier
             if constexpr (PF2) {
                 const int64_t n2 = (cb + 2 * CB < b1) ? cb + 2 * CB : nx;
-                const u32x4 *src_ = Q8 + (nx / 4) * VT * 16;
-#pragma unroll
-                for (int u = 0; u < NQ; u++) {
-                    const int it_ = tid + NT * u;
-                    if (it_ < NQT) rq[u] = src_[it_];
-                }
 #pragma unroll
                 for (int s = 0; s < ST; s++)
 #pragma unroll
@@ -663,7 +686,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
                             packed + (base + s * 16 + r) * ldp + n2 + (CB / 4) * g + 16 * n);
                     }
             } else {
-                DPCA_ZA8_LOAD(nx);
+                DPCA_ZA8_LOADP(nx);
             }
         }
         // The chunk's B tiles (K step t, tile v) form one sequence i = t VT + v
@@ -712,11 +735,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
             // bunch the VALU work); only SALU moves across
             __builtin_amdgcn_sched_barrier(0x4);
         }
-        DPCA_ZA8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
+        lds_dma_wait();
         __syncthreads();
     }
-#undef DPCA_ZA8_STORE
-#undef DPCA_ZA8_LOAD
+#undef DPCA_ZA8_LOADP
+#undef DPCA_ZA8_LOADQ
     float *out = Tpart + (int64_t)by * Npad * KP;
 #pragma unroll
     for (int oc = 0; oc < CT; oc++) {
