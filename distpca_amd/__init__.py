@@ -32,7 +32,7 @@ class _Params(ctypes.Structure):
         ("prefix", ctypes.c_char * 1024), ("precision", ctypes.c_int),
         ("device", ctypes.c_int), ("device_cap", ctypes.c_longlong),
         ("strict_passes", ctypes.c_int), ("host_mode", ctypes.c_int),
-        ("slices", ctypes.c_int),
+        ("slices", ctypes.c_int), ("zt_image", ctypes.c_int),
     ]
 
 
@@ -49,6 +49,7 @@ class Report(ctypes.Structure):
         ("t_h2d", ctypes.c_double), ("h2d_bytes", ctypes.c_double),
         ("host_mode", ctypes.c_int), ("qr_repaired", ctypes.c_int),
         ("resident_bytes", ctypes.c_double), ("graph_its", ctypes.c_int),
+        ("zt_image", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -191,13 +192,14 @@ class Params:
     strict_passes: int = 1
     host_mode: int = HOST_PINNED
     slices: int = 0
+    zt_image: int = 1
 
     def c(self) -> _Params:
         p = _Params()
         lib().dpca_params_init(ctypes.byref(p))
         for f in ("nsv", "nrhs", "power", "crit", "tol", "bsize", "miter", "verbose",
                   "fwrite", "memory", "precision", "device", "device_cap", "strict_passes", "host_mode",
-                  "slices"):
+                  "slices", "zt_image"):
             setattr(p, f, getattr(self, f))
         p.prefix = self.prefix.encode()
         return p

@@ -9,7 +9,8 @@
 // of an iteration as the next iteration's first: power passes per iteration
 // instead of power+1), -hostmem pinned|pageable|file (where a streamed matrix
 // lives between passes; file re-reads the .bed every pass), -slices 2|3 (int8
-// slices of the mixed form's FP64 operand).
+// slices of the mixed form's FP64 operand), -ztimage 0 (no tile-order copy of
+// the matrix for the mixed form's Z^T T kernel).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -210,6 +211,8 @@ int main(int argc, char **argv)
     findarg("fullSVD", INT, &p.fullsvd, argc, argv);
     findarg("device", INT, &p.device, argc, argv);
     findarg("slices", INT, &p.slices, argc, argv);
+    int zti = 1;
+    if (findarg("ztimage", INT, &zti, argc, argv)) p.zt_image = zti != 0;
     int strict = 1;
     if (findarg("strict", INT, &strict, argc, argv)) p.strict_passes = strict != 0;
     double devcap_gb = 0;

@@ -164,6 +164,7 @@ struct dpca_ctx {
     std::vector<std::string> iids;
     // device buffers
     uint8_t *packed = nullptr;        // Npad x ldp
+    uint8_t *zt_img = nullptr;        // its tile-order image for Z^T T (mixed, resident)
     double *zval = nullptr, *mu = nullptr, *dsc = nullptr;
     int64_t *ssum = nullptr, *sctr = nullptr;
     double *Q = nullptr, *ARHS = nullptr, *RHS2 = nullptr, *RHS2old = nullptr, *tmp = nullptr;
@@ -306,7 +307,8 @@ static void free_ctx(dpca_ctx *c)
     (void)hipSetDevice(c->dev);
     void *ptrs[] = {c->packed, c->zval, c->mu, c->dsc, c->ssum, c->sctr, c->Q, c->ARHS,
                     c->RHS2, c->RHS2old, c->tmp, c->T, c->part, c->gpart, c->G, c->Rinv,
-                    c->w, c->X, c->dots, c->flags, c->cinfo, c->its, c->UW8, c->Q8, c->qmax, c->smax, c->Tpart, c->Apart, c->Tseg};
+                    c->w, c->X, c->dots, c->flags, c->cinfo, c->its, c->UW8, c->Q8, c->qmax, c->smax, c->Tpart, c->Apart, c->Tseg,
+                    c->zt_img};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->resident) (void)hipFree(c->resident);
@@ -555,11 +557,28 @@ static int block_stats(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows
     return 0;
 }
 
+// Z^T T reads a tile-order image of the matrix (a second copy of the same
+// size, launch_zt_image) in mixed, HBM-resident contexts that are not
+// serving jobs, with 8-wave Z^T T workgroups, unless p.zt_image is 0; a
+// device_cap has to hold both copies.
+static bool want_zt_image(const dpca_ctx *c, int64_t bytes)
+{
+    return c->p.zt_image && c->p.precision == DPCA_MIXED && !c->streamed && !g_job_ctx &&
+           c->zt_waves == 8 && (c->p.device_cap <= 0 || 2 * bytes <= c->p.device_cap);
+}
+
+// Once the matrix is resident: per-SNP statistics and, when allocated, the
+// Z^T T image.
 static int compute_stats(dpca_ctx *c)
 {
     if (c->streamed) return 0;  // computed on the first streamed pass
     int r = block_stats(c, c->packed, 0, c->Npad);
     if (!r) c->stats_done = true;
+    if (!r && c->zt_img) {
+        Stage s(c, T_LOAD);
+        DPCA_CHECK(launch_zt_image(c->st, c->packed, c->Npad, c->ldp, c->sa_tiles, c->zt_waves,
+                                   c->zt_img));
+    }
     return r;
 }
 
@@ -687,6 +706,15 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
     if (!c->streamed && (r = dalloc(&c->packed, (size_t)bytes))) {
         free_ctx(c);
         return r;
+    }
+    if (want_zt_image(c, bytes)) {
+        // only where the copy leaves 1 GB of the device free
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > (size_t)bytes + (1ull << 30) &&
+            (r = dalloc(&c->zt_img, (size_t)bytes))) {
+            free_ctx(c);
+            return r;
+        }
     }
     if (c->streamed) {
         // The rest of the cap keeps the leading blocks in HBM once the first
@@ -942,9 +970,11 @@ static int pass_block(dpca_ctx *c, const uint8_t *blk, int64_t r0, int64_t rows,
     }
     {
         Stage s(c, T_MM_AT);
-        DPCA_CHECK(launch_zt_i8(c->st, blk, rows, c->ldp, UW8, c->smax, c->k, c->ns, c->sa_tiles,
-                                c->zt_waves,
-                                c->rps, nsplit, c->Apart));
+        // the image covers the resident matrix (rows from r0 at the same offset)
+        const uint8_t *img =
+            (c->zt_img && blk == c->packed + r0 * c->ldp) ? c->zt_img + r0 * c->ldp : nullptr;
+        DPCA_CHECK(launch_zt_i8(c->st, blk, img, rows, c->ldp, UW8, c->smax, c->k, c->ns,
+                                c->sa_tiles, c->zt_waves, c->rps, nsplit, c->Apart));
     }
     Stage s(c, T_MM);
     DPCA_CHECK(launch_reduce_f32(c->st, c->Apart, nsplit, c->Mpad * c->kp, out,
@@ -1319,6 +1349,7 @@ static int run_once(dpca_ctx *c, double *rhs, double *sigma, dpca_report *rep, d
         rep->qr_repaired = c->qr_repairs;
         rep->resident_bytes = c->streamed ? (double)(c->res_blocks * c->chunk_rows * c->ldp) : 0.0;
         rep->graph_its = c->graph_its;
+        rep->zt_image = c->zt_img != nullptr;
     }
     return 0;
 }
@@ -1406,6 +1437,7 @@ int dpca_params_init(dpca_params *p)
     p->strict_passes = 1;
     p->host_mode = DPCA_HOST_PINNED;
     p->slices = 0;
+    p->zt_image = 1;
     return 0;
 }
 
@@ -2177,6 +2209,7 @@ int dpca_footprint(int M, long long N, const dpca_params *p, long long *bytes)
     g_dry = &total;
     r = alloc_work(&c);
     g_dry = nullptr;
+    if (!r && want_zt_image(&c, packed)) total += (size_t)packed;
     *bytes = (long long)total;
     return r;
 }

@@ -143,9 +143,15 @@ inline ColTiles col_tiles(int k, int ns)
 hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int ns, int kp,
                       const double *mu, const double *dsc, int64_t rps, unsigned *smax, void *UW8,
                       bool have_max);
-hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const void *UW8, const unsigned *smax, int k, int ns, int sa_tiles,
-                        int waves, int64_t rows_per_split, int nsplit, float *part);
+// img: the tile-order image of the same rows (launch_zt_image, 8-wave
+// workgroups), or null to stage the codes from packed.
+hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, const uint8_t *img, int64_t rows,
+                        int64_t ldp, const void *UW8, const unsigned *smax, int k, int ns,
+                        int sa_tiles, int waves, int64_t rows_per_split, int nsplit, float *part);
+// Tile-order image of rows x ldp packed bytes for k_zt_i8's DMA form (rows x
+// ldp bytes at img; rows a multiple of 128, 8-wave workgroups).
+hipError_t launch_zt_image(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                           int sa_tiles, int waves, uint8_t *img);
 // Z Q on int8 MFMA: codes as exact h / m rows, Q as ns int8 slices per
 // (sample segment of 4*seg_bytes samples, column) with a power-of-two scale
 // from qmax.  Q8: Mpad/16 x vt x 16 words of 16 bytes; qmax: nseg x kp.

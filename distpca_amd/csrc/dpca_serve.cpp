@@ -64,7 +64,7 @@ hipError_t launch_projection(hipStream_t st, const dpca_model *m, const ProjPlan
                           (double *)part);
         if (e == hipSuccess) e = launch_reduce_parts(st, (const double *)part, pl.splits, n, scores, 0);
     } else {
-        e = launch_zt_i8(st, batch, m->Npad, pl.ldb, m->UW8, m->smax, m->k, m->ns, pl.sa_tiles, 8,
+        e = launch_zt_i8(st, batch, nullptr, m->Npad, pl.ldb, m->UW8, m->smax, m->k, m->ns, pl.sa_tiles, 8,
                          pl.rps, pl.nsplit, (float *)part);
         if (e == hipSuccess) e = launch_reduce_f32(st, (const float *)part, pl.nsplit, n, scores, 0);
     }

@@ -15,6 +15,7 @@
 #include "dpca_internal.h"
 #include "k_common.h"
 
+#include <algorithm>
 #include <mutex>
 
 namespace dpca {
@@ -307,13 +308,52 @@ hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int 
     return hipGetLastError();
 }
 
+// Tile-order image of the packed matrix for k_zt_i8 (IMG form): for each
+// 128-SNP chunk c and TB-byte sample tile x, one block of TB rows (byte
+// columns) x 32 dwords, block index c * (ldp / TB) + x; dword `col` of row b
+// holds byte column x TB + b of SNPs 4G..4G+3 (byte j = SNP 4G + j) with
+// G = col ^ 8 (b & 3).  That is the kernel's LDS tile itself, so a chunk
+// reaches LDS as one contiguous DMA.  The swizzle spreads the 4 rows x 4
+// pairs of dwords one paired read touches over distinct banks.
+// One workgroup (256 threads) per block: the block's 128 x TB input bytes
+// are staged in LDS by coalesced row loads, then written in block order.
+template <int TB>
+__global__ __launch_bounds__(256) void k_zt_image(const uint8_t *__restrict__ packed, int64_t ldp,
+                                                  int64_t nblk, uint32_t *__restrict__ img)
+{
+    __shared__ uint32_t in[128][TB / 4 + 1];
+    const int tid = threadIdx.x;
+    const int64_t tiles = ldp / TB;
+    for (int64_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+        const int64_t c = blk / tiles, x = blk % tiles;
+        const uint8_t *src = packed + c * 128 * ldp + x * TB;
+        __syncthreads();  // the previous block's reads of `in` are done
+        for (int i = tid; i < 128 * (TB / 4); i += 256) {
+            const int row = i / (TB / 4), q = i % (TB / 4);
+            in[row][q] = *reinterpret_cast<const uint32_t *>(src + row * ldp + 4 * q);
+        }
+        __syncthreads();
+        uint32_t *dst = img + blk * (TB * 32);
+        for (int o = tid; o < TB * 32; o += 256) {
+            const int b = o / 32, col = o % 32;
+            const int G = col ^ (8 * (b & 3));
+            uint32_t w = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) w |= ((in[4 * G + j][b / 4] >> (8 * (b & 3))) & 0xFFu) << (8 * j);
+            dst[o] = w;
+        }
+    }
+}
+
 // Workgroup: WAVES waves x SA tiles x 16 samples (8 x 4: 512 samples = 128
 // packed bytes per row) and one SNP split (blockIdx.y); 128-SNP chunks of
 // packed bytes staged in LDS transposed to [byte column][4-SNP group], so a
 // lane reads the codes of 4 consecutive SNPs for its sample with one dword
-// load, and the chunk's UW8 words copied to LDS.
-template <int NS, int SA, int F, int RT, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict__ packed, int64_t ldp,
+// load, and the chunk's UW8 words copied to LDS.  IMG: the chunk's codes come
+// from the tile-order image (img, above) by DMA, with no register staging.
+template <int NS, int SA, int F, int RT, int WAVES, bool IMG>
+__global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict__ packed,
+                                               const uint8_t *__restrict__ img, int64_t ldp,
                                                const u32x4 *__restrict__ UW8,
                                                const unsigned *__restrict__ smax, int64_t rows,
                                                int64_t rows_per_split, int64_t Mpad, int R,
@@ -332,8 +372,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     // dwords: both the transposing stores (one group, many rows per
     // half-wave) and the paired reads (4 rows x 2 groups) then touch
     // distinct banks.  Groups 2x, 2x+1 stay one aligned 8-byte pair, in
-    // swapped order when row / 4 is odd.
-    constexpr int PKW = SC / 4 + 4;
+    // swapped order when row / 4 is odd.  IMG: rows of 32 dwords in the
+    // image's order (pairs never swapped).
+    constexpr int PKW = IMG ? SC / 4 : SC / 4 + 4;
     static_assert(SA % 2 == 0, "row / 4 parity must follow the tile index");
     __shared__ uint32_t pk[2][TB][PKW];
     __shared__ u32x4 uw[2][NUWT];
@@ -410,11 +451,28 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         }                                                                                   \
     } while (0)
     static_assert(NUWT % 64 == 0, "whole waves of UW8 words");
+    // IMG: a chunk's block of the image (TB rows x 32 dwords) into pk[buf]
+    constexpr int NIW = TB * SC / 16;  // 16-byte words per block (TB x SC bytes)
+    static_assert(NIW % 64 == 0, "whole waves of image words");
+    const uint32_t pk_lds = lds_addr(&pk[0][0][0]);
+#define DPCA_ZT8_LOADI(c0, buf)                                                             \
+    do {                                                                                    \
+        const uint8_t *b_ = img + (((c0) / SC) * (ldp / TB) + bx) * (int64_t)(TB * SC);     \
+        _Pragma("unroll") for (int u = 0; u < (NIW + NT - 1) / NT; u++) {                  \
+            const int w_ = NT * u + 64 * wave_u;                                            \
+            if (w_ < NIW)                                                                   \
+                lds_dma16(pk_lds + 16u * (uint32_t)((buf) * NIW + w_), b_ + 16 * (w_ + lane)); \
+        }                                                                                   \
+    } while (0)
     {
         const int64_t c0 = sbeg < send ? sbeg : 0;
-        DPCA_ZT8_LOADP(c0, rpk);
+        if constexpr (IMG) {
+            DPCA_ZT8_LOADI(c0, 0);
+        } else {
+            DPCA_ZT8_LOADP(c0, rpk);
+        }
         DPCA_ZT8_LOADU(c0, 0);
-        DPCA_ZT8_STORE(0);
+        if constexpr (!IMG) DPCA_ZT8_STORE(0);
     }
     lds_dma_wait();
     __syncthreads();
@@ -424,8 +482,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         {
             // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
-            DPCA_ZT8_LOADP(nx, rpk);
-            DPCA_ZT8_LOADU(nx, cur ^ 1);  // last read before the previous barrier
+            if constexpr (IMG) {
+                DPCA_ZT8_LOADI(nx, cur ^ 1);  // last read before the previous barrier
+            } else {
+                DPCA_ZT8_LOADP(nx, rpk);
+            }
+            DPCA_ZT8_LOADU(nx, cur ^ 1);
         }
         // The chunk's B tiles (K step t, tile v) form one sequence i = t VT + v
         // read from LDS ZB tiles ahead of their MFMAs, and the packed words
@@ -444,8 +506,13 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
 #pragma unroll
             for (int a = 0; a < SA; a++) {
                 const int i = wave * SA * 16 + a * 16 + r;  // sample within the tile
-                const int sw = (wave * SA + a) & 30;         // (i / 16) & ~1
-                wq[a] = *reinterpret_cast<const uint2 *>(&pk[cur][i >> 2][(8 * t + 2 * g) ^ sw]);
+                if constexpr (IMG) {
+                    wq[a] = *reinterpret_cast<const uint2 *>(
+                        &pk[cur][i >> 2][(8 * t + 2 * g) ^ (8 * ((i >> 2) & 3))]);
+                } else {
+                    const int sw = (wave * SA + a) & 30;     // (i / 16) & ~1
+                    wq[a] = *reinterpret_cast<const uint2 *>(&pk[cur][i >> 2][(8 * t + 2 * g) ^ sw]);
+                }
             }
         };
         // A operands of K step t + 1 are built from its packed words (read at
@@ -455,8 +522,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         u32x4 av[2][SA];
         auto build = [&](int t, int a) {
             const int sh = 2 * (r & 3);                      // sample i & 3 = r & 3
-            const uint32_t x0 = (a & 1) ? wq[a].y : wq[a].x;   // group 8t + 2g
-            const uint32_t x1 = (a & 1) ? wq[a].x : wq[a].y;   // group 8t + 2g + 1
+            const bool sw = !IMG && (a & 1);
+            const uint32_t x0 = sw ? wq[a].y : wq[a].x;   // group 8t + 2g
+            const uint32_t x1 = sw ? wq[a].x : wq[a].y;   // group 8t + 2g + 1
             const uint32_t c0 = (x0 >> sh) & 0x03030303u;
             const uint32_t c1 = (x1 >> sh) & 0x03030303u;
             av[t & 1][a] = u32x4{__builtin_amdgcn_perm(0u, HT, c0), __builtin_amdgcn_perm(0u, HT, c1),
@@ -483,11 +551,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
             // bunch the VALU work); only SALU moves across
             __builtin_amdgcn_sched_barrier(0x4);
         }
-        DPCA_ZT8_STORE(cur ^ 1);  // the other buffer: last read before the previous barrier
+        if constexpr (!IMG) DPCA_ZT8_STORE(cur ^ 1);  // last read before the previous barrier
         lds_dma_wait();
         __syncthreads();
     }
 #undef DPCA_ZT8_STORE
+#undef DPCA_ZT8_LOADI
 #undef DPCA_ZT8_LOADU
 #undef DPCA_ZT8_LOADP
     float *out = part + ((int64_t)by * Mpad + 4 * mb + wave * SA * 16) * KP;
@@ -785,15 +854,23 @@ constexpr bool za_fits(int st, int vt, int waves)
 constexpr int zt_tile_bytes(int sa, int waves) { return 4 * sa * waves; }
 
 template <int NS, int SA, int F, int RT, int W>
-static hipError_t zt_go(hipStream_t st, dim3 grid, const uint8_t *packed, int64_t ldp,
-                        const u32x4 *UW8, const unsigned *smax, int64_t rows, int64_t rps,
-                        int64_t Mpad, int R, float *part)
+static hipError_t zt_go(hipStream_t st, dim3 grid, const uint8_t *packed, const uint8_t *img,
+                        int64_t ldp, const u32x4 *UW8, const unsigned *smax, int64_t rows,
+                        int64_t rps, int64_t Mpad, int R, float *part)
 {
     if constexpr (!zt_fits(SA, NS * F + RT, W)) {
         return hipErrorInvalidValue;  // would not fit the registers
     } else {
-        k_zt_i8<NS, SA, F, RT, W><<<grid, 64 * W, 0, st>>>(packed, ldp, UW8, smax, rows, rps, Mpad,
-                                                          R, part);
+        if constexpr (W == 8) {  // the image form exists for 8-wave workgroups
+            if (img) {
+                k_zt_i8<NS, SA, F, RT, W, true><<<grid, 64 * W, 0, st>>>(
+                    packed, img, ldp, UW8, smax, rows, rps, Mpad, R, part);
+                return hipGetLastError();
+            }
+        }
+        if (img) return hipErrorInvalidValue;
+        k_zt_i8<NS, SA, F, RT, W, false><<<grid, 64 * W, 0, st>>>(packed, nullptr, ldp, UW8, smax,
+                                                                 rows, rps, Mpad, R, part);
         return hipGetLastError();
     }
 }
@@ -825,7 +902,7 @@ template <int NS, int SA, int F, int RT, int W>
 static int zt_occ()
 {
     if constexpr (!zt_fits(SA, NS * F + RT, W)) return 0;
-    else return blocks_per_cu(k_zt_i8<NS, SA, F, RT, W>, 64 * W);
+    else return blocks_per_cu(k_zt_i8<NS, SA, F, RT, W, false>, 64 * W);
 }
 
 template <int NS, int ST, int F, int RT, int W>
@@ -951,14 +1028,33 @@ int zt_i8_blocks_per_cu(int k, int ns, int sa_tiles, int waves)
 
 int64_t zt_i8_tile_bytes(int sa_tiles, int waves) { return zt_tile_bytes(sa_tiles, waves); }
 
+// Tile-order image of rows x ldp packed bytes for the Z^T T workgroups of
+// sa_tiles x waves (8 waves only): rows x ldp bytes at img.
+hipError_t launch_zt_image(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
+                           int sa_tiles, int waves, uint8_t *img)
+{
+    const int tb = zt_tile_bytes(sa_tiles, waves);
+    if (waves != 8 || rows % 128 || ldp % tb || !img) return hipErrorInvalidValue;
+    const int64_t nblk = rows / 128 * (ldp / tb);
+    const unsigned grid = (unsigned)std::min<int64_t>(nblk, 1 << 16);
+    uint32_t *out = reinterpret_cast<uint32_t *>(img);
+    if (tb == 64) k_zt_image<64><<<grid, 256, 0, st>>>(packed, ldp, nblk, out);
+    else if (tb == 128) k_zt_image<128><<<grid, 256, 0, st>>>(packed, ldp, nblk, out);
+    else if (tb == 256) k_zt_image<256><<<grid, 256, 0, st>>>(packed, ldp, nblk, out);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 // ldp a multiple of the tile's byte width, rows and rows_per_split
-// multiples of 128, rows_per_split at most 2^22 (int32 sums).
-hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int64_t ldp,
-                        const void *UW8, const unsigned *smax, int k, int ns, int sa_tiles,
-                        int waves, int64_t rows_per_split, int nsplit, float *part)
+// multiples of 128, rows_per_split at most 2^22 (int32 sums).  img: the
+// tile-order image (launch_zt_image, same sa_tiles and waves), or null to
+// stage the codes from packed.
+hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, const uint8_t *img, int64_t rows,
+                        int64_t ldp, const void *UW8, const unsigned *smax, int k, int ns,
+                        int sa_tiles, int waves, int64_t rows_per_split, int nsplit, float *part)
 {
     if (!zt_i8_valid(k, ns, sa_tiles, ldp, waves) || rows % 128 || rows_per_split % 128 ||
-        rows_per_split > (1 << 22))
+        rows_per_split > (1 << 22) || (img && waves != 8))
         return hipErrorInvalidValue;
     const ColTiles ct = col_tiles(k, ns);
     // 1-D grid of tiles x splits, ordered per XCD inside the kernel
@@ -967,7 +1063,7 @@ hipError_t launch_zt_i8(hipStream_t st, const uint8_t *packed, int64_t rows, int
     const u32x4 *uw = reinterpret_cast<const u32x4 *>(UW8);
 #define ZT_T(NS_, SA_, F_, RT_, W_)                                                         \
     if (sa_tiles == SA_ && waves == W_)                                                      \
-        return zt_go<NS_, SA_, F_, RT_, W_>(st, grid, packed, ldp, uw, smax, rows,           \
+        return zt_go<NS_, SA_, F_, RT_, W_>(st, grid, packed, img, ldp, uw, smax, rows,      \
                                             rows_per_split, Mpad, ct.R, part);
 #define X(NS_, F_, RT_)                                                                     \
     if (ct.NS == NS_ && ct.F == F_ && ct.RT == RT_) {                                        \

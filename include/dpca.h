@@ -60,6 +60,13 @@ typedef struct dpca_params {
                                each product, 2 or 3 (0 = the default, 3);
                                2 issues a third fewer MFMAs at ~100x the
                                pass error of 3                               */
+    int zt_image;           /* mixed form, HBM-resident matrix: 1 (default)
+                               keeps a second copy of the packed matrix in
+                               the tile order of the Z^T T kernel, which then
+                               reaches LDS by DMA (results bitwise the same
+                               as without it); used where device_cap (if set)
+                               holds both copies and the device has the room.
+                               0: never                                      */
 } dpca_params;
 
 enum { DPCA_HOST_PINNED = 0, DPCA_HOST_PAGEABLE = 1, DPCA_HOST_FILE = 2 };
@@ -89,6 +96,7 @@ typedef struct dpca_report {
                                graphs (resident contexts; the stage timers
                                of those are split in the proportions of the
                                eager first iteration)                        */
+    int zt_image;           /* 1 when Z^T T read the tile-order image        */
 } dpca_report;
 
 /* Defaults of initialize_structure (src/utilities.cpp:195-232). */

@@ -105,14 +105,21 @@ def test_cli_parameter_errors_before_any_device_call(tiny_bfile, args, msg):
 
 def test_footprint_without_device():
     # the HBM footprint of a context is host arithmetic (no device call): the
-    # resident 100k x 1M matrix plus its work buffers; above device_cap the
-    # two block buffers and the resident blocks stay within the cap
+    # resident 100k x 1M matrix plus its work buffers (the mixed form by
+    # default also the matrix's tile-order copy for Z^T T); above device_cap
+    # the two block buffers and the resident blocks stay within the cap
     M, N = 100000, 1000000
     packed = -(-N // 1024) * 1024 * (-(-((M + 3) // 4) // 256) * 256)
-    mixed = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.MIXED))
+    mixed = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.MIXED, zt_image=0))
+    mixed_img = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.MIXED))
     fp64 = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.FP64))
     assert packed < mixed < packed + (3 << 30)
+    assert mixed_img == mixed + packed
     assert packed < fp64 < packed + (3 << 30)
+    # a device_cap that holds one copy but not two: no image
+    one = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.MIXED,
+                                     device_cap=packed + (1 << 30)))
+    assert one == mixed
     cap = 2 * 10**9
     streamed = D.footprint(M, N, D.Params(nsv=20, nrhs=40, power=6, precision=D.MIXED,
                                           device_cap=cap))

@@ -87,6 +87,54 @@ def test_mixed_pass_tiles_per_wave(slices, zt, za, monkeypatch):
     assert err < PASS_TOL[slices], err
 
 
+@pytest.mark.parametrize("slices,k,zt", [(3, 40, 0), (3, 20, 2), (3, 20, 4), (2, 40, 0),
+                                         (2, 20, 2), (3, 33, 0), (3, 64, 0)])
+def test_zt_image_bitwise(slices, k, zt, monkeypatch):
+    # Z^T T from the matrix's tile-order copy (by DMA) against Z^T T staging
+    # the codes itself: the same integer sums, so passes and whole runs are
+    # bitwise equal; sample tiles of 64 and 128 bytes (zt = 2, 4 sample tiles
+    # per wave; 0 = the launch plan's choice)
+    M, N = 1500, 6000
+    print(f"[gpu] zt image slices={slices} k={k} zt={zt}", flush=True)
+    if zt:
+        monkeypatch.setenv("DPCA_ZT_TILES", str(zt))
+    packed = _packed_random(M, N, seed=11)
+    Q = np.linalg.qr(np.random.default_rng(11).standard_normal((M, k)))[0]
+    out = {}
+    for img in (0, 1):
+        p = D.Params(nsv=4, nrhs=k, power=2, miter=3, bsize=512, precision=D.MIXED,
+                     slices=slices, zt_image=img)
+        with D.PCA.from_packed(packed, M, N, p) as pca:
+            a = pca.pass_(Q)
+            b = pca.pass_(a / np.linalg.norm(a, axis=0))
+            sigma, rhs, rep = pca.run()
+        assert rep["zt_image"] == img
+        out[img] = (a, b, sigma, rhs)
+    for x, y in zip(out[0], out[1]):
+        assert np.array_equal(x.view(np.int64), y.view(np.int64))
+
+
+def test_zt_image_not_used(monkeypatch):
+    # no copy for a streamed matrix, a device_cap that holds one copy only,
+    # the FP64 form or 4-wave workgroups; one where the cap holds both
+    M, N = 1000, 8192
+    print("[gpu] zt image: when it is not used", flush=True)
+    packed = _packed_random(M, N, seed=12)
+    nbytes = 8192 * 256  # Npad x ldp
+    cases = [(dict(precision=D.MIXED, device_cap=nbytes // 2), None, 0),
+             (dict(precision=D.MIXED, device_cap=nbytes + (nbytes >> 1)), None, 0),
+             (dict(precision=D.MIXED, device_cap=2 * nbytes), None, 1),
+             (dict(precision=D.FP64), None, 0),
+             (dict(precision=D.MIXED), "4", 0)]
+    for kw, waves, used in cases:
+        if waves:
+            monkeypatch.setenv("DPCA_I8_WAVES", waves)
+        p = D.Params(nsv=2, nrhs=8, miter=1, bsize=1024, **kw)
+        with D.PCA.from_packed(packed, M, N, p) as pca:
+            _, _, rep = pca.run()
+        assert rep["zt_image"] == used, (kw, waves, rep["zt_image"])
+
+
 def test_mixed_c1(c1_golden):
     print("[gpu] mixed c1 run", flush=True)
     spec = D.gen_spec(1024, 50000, npop=3, fst=0.05, missing=0.01, seed=0)
