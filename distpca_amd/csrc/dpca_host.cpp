@@ -708,12 +708,13 @@ static int new_ctx(const dpca_params *p, int M, int64_t N, dpca_ctx **out, bool 
         return r;
     }
     if (want_zt_image(c, bytes)) {
-        // only where the copy leaves 1 GB of the device free
+        // only where the copy leaves 1 GB of the device free; without the
+        // memory the context simply runs the staging kernel
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > (size_t)bytes + (1ull << 30) &&
-            (r = dalloc(&c->zt_img, (size_t)bytes))) {
-            free_ctx(c);
-            return r;
+            hipMalloc((void **)&c->zt_img, (size_t)bytes) != hipSuccess) {
+            c->zt_img = nullptr;
+            (void)hipGetLastError();
         }
     }
     if (c->streamed) {

@@ -135,6 +135,25 @@ def test_zt_image_not_used(monkeypatch):
         assert rep["zt_image"] == used, (kw, waves, rep["zt_image"])
 
 
+def test_cli_ztimage_flag(tmp_path):
+    # DistPCA.exe -precision mixed with and without the tile-order copy
+    # (-ztimage 0) writes the same files, byte for byte
+    import subprocess
+    pre = str(tmp_path / "zti")
+    D.write_synthetic(pre, D.gen_spec(800, 6000, npop=3, fst=0.05, missing=0.01, seed=9))
+    print("[gpu] CLI -ztimage", flush=True)
+    outs = {}
+    for tag, extra in (("img", []), ("noimg", ["-ztimage", "0"])):
+        res = subprocess.run([D.EXE_PATH, "-bfile", pre, "-nsv", "4", "-nrhs", "8", "-power", "2",
+                              "-bsize", "1000", "-fwrite", "1", "-precision", "mixed",
+                              "-prefix", str(tmp_path / tag)] + extra,
+                             capture_output=True, text=True, timeout=300)
+        assert res.returncode == 0, res.stdout + res.stderr
+        outs[tag] = [open(str(tmp_path / (tag + f))).read()
+                     for f in ("_singularValues.txt", "_leftSingularVectors.txt")]
+    assert outs["img"] == outs["noimg"]
+
+
 def test_mixed_c1(c1_golden):
     print("[gpu] mixed c1 run", flush=True)
     spec = D.gen_spec(1024, 50000, npop=3, fst=0.05, missing=0.01, seed=0)
