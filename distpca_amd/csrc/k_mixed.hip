@@ -134,6 +134,14 @@ __device__ __forceinline__ void lds_dma16(uint32_t lds, const void *src)
 }
 // all of this wave's global loads, the LDS copies among them, have landed
 __device__ __forceinline__ void lds_dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+// all but this wave's N most recently issued global loads have landed (the
+// count drains in issue order)
+template <int N>
+__device__ __forceinline__ void lds_dma_wait_all_but()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt field");
+    asm volatile("s_waitcnt vmcnt(%0)" : : "n"(N) : "memory");
+}
 // LDS byte address of a __shared__ object (the low half of its generic address)
 __device__ __forceinline__ uint32_t lds_addr(const void *p) { return (uint32_t)(uintptr_t)p; }
 
@@ -375,9 +383,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     // swapped order when row / 4 is odd.  IMG: rows of 32 dwords in the
     // image's order (pairs never swapped).
     constexpr int PKW = IMG ? SC / 4 : SC / 4 + 4;
+    // IMG, where the registers hold one workgroup per CU anyway and three
+    // buffers fit the LDS: three buffers, chunk i+2's copies in flight while
+    // chunk i is multiplied (twice the time to cover HBM latency)
+    constexpr int BUFB = TB * PKW * 4 + NUWT * 16;  // LDS bytes per buffer
+    constexpr int NBUF = (IMG && SA * VT >= 24 && 3 * BUFB <= 160 * 1024) ? 3 : 2;
     static_assert(SA % 2 == 0, "row / 4 parity must follow the tile index");
-    __shared__ uint32_t pk[2][TB][PKW];
-    __shared__ u32x4 uw[2][NUWT];
+    __shared__ uint32_t pk[NBUF][TB][PKW];
+    __shared__ u32x4 uw[NBUF][NUWT];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
     int bx, by;  // sample tile, SNP split
@@ -420,14 +433,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         }                                                                                   \
     } while (0)
     // lane l of wave w copies word NT u + 64 w + l of the chunk's UW8 words
-    // (NUWT is a multiple of 64, so each wave's test is uniform)
+    // (NUWT is a multiple of 64, so each wave's test is uniform).  Three
+    // buffers: every wave issues NUW copies (one past the end copies word
+    // w mod NUWT again, the same bytes to the same place), so that its count
+    // of copies in flight is a constant.
     const uint32_t uw_lds = lds_addr(&uw[0][0]);
     const int wave_u = __builtin_amdgcn_readfirstlane(wave);
-#define DPCA_ZT8_LOADU(c0, buf)                                                             \
+    __builtin_assume(wave_u >= 0 && wave_u < WAVES);
+#define DPCA_ZT8_LOADU(c0, buf)                                                          \
     do {                                                                                    \
         const u32x4 *q_ = UW8 + ((c0) / 8) * VT * 16;                                       \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
-            const int w_ = NT * u + 64 * wave_u;                                            \
+            int w_ = NT * u + 64 * wave_u;                                                  \
+            if (NBUF == 3) w_ %= NUWT;                                                      \
             if (w_ < NUWT)                                                                  \
                 lds_dma16(uw_lds + 16u * (uint32_t)((buf) * NUWT + w_), q_ + w_ + lane);    \
         }                                                                                   \
@@ -464,6 +482,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
                 lds_dma16(pk_lds + 16u * (uint32_t)((buf) * NIW + w_), b_ + 16 * (w_ + lane)); \
         }                                                                                   \
     } while (0)
+    // copies per wave and chunk (three buffers: both counts uniform)
+    constexpr int NCP = (NIW + NT - 1) / NT + NUW;
+    static_assert(NBUF == 2 || NIW % NT == 0, "uniform image copies");
     {
         const int64_t c0 = sbeg < send ? sbeg : 0;
         if constexpr (IMG) {
@@ -473,13 +494,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         }
         DPCA_ZT8_LOADU(c0, 0);
         if constexpr (!IMG) DPCA_ZT8_STORE(0);
+        if constexpr (NBUF == 3) {
+            const int64_t c1 = (c0 + SC < send) ? c0 + SC : c0;
+            DPCA_ZT8_LOADI(c1, 1);
+            DPCA_ZT8_LOADU(c1, 1);
+        }
     }
-    lds_dma_wait();
+    if constexpr (NBUF == 3) lds_dma_wait_all_but<NCP>();  // chunk 1 may be in flight
+    else lds_dma_wait();
     __syncthreads();
 
     int cur = 0;
-    for (int64_t s0 = sbeg; s0 < send; s0 += SC, cur ^= 1) {
-        {
+    for (int64_t s0 = sbeg; s0 < send; s0 += SC, cur = (cur + 1 == NBUF) ? 0 : cur + 1) {
+        if constexpr (NBUF == 3) {
+            // chunk i+2 (the last chunk again near the end) into the buffer
+            // chunk i-1 was read from before the previous barrier
+            const int64_t n2 = (s0 + 2 * SC < send) ? s0 + 2 * SC : send - SC;
+            const int nb = (cur == 0) ? 2 : cur - 1;
+            DPCA_ZT8_LOADI(n2, nb);
+            DPCA_ZT8_LOADU(n2, nb);
+        } else {
             // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (s0 + SC < send) ? s0 + SC : s0;
             if constexpr (IMG) {
@@ -552,7 +586,8 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
             __builtin_amdgcn_sched_barrier(0x4);
         }
         if constexpr (!IMG) DPCA_ZT8_STORE(cur ^ 1);  // last read before the previous barrier
-        lds_dma_wait();
+        if constexpr (NBUF == 3) lds_dma_wait_all_but<NCP>();  // chunk i+1 landed
+        else lds_dma_wait();
         __syncthreads();
     }
 #undef DPCA_ZT8_STORE
