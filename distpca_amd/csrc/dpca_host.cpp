@@ -1855,7 +1855,10 @@ int dpca_run(dpca_ctx *c, double *rhs_inout, double *sigma, dpca_report *report)
 
 int dpca_crit_history(dpca_ctx *c, double *vals, int cap)
 {
-    if (!c) return -1;
+    if (!c || cap < 0 || (cap > 0 && !vals)) {
+        set_error("dpca_crit_history: bad arguments");
+        return -1;
+    }
     int n = std::min<int>(cap, (int)c->crit_hist.size());
     for (int i = 0; i < n; i++) vals[i] = c->crit_hist[i];
     return n;
