@@ -683,14 +683,23 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
     constexpr int VT = NS * F + RT;
     constexpr int CT = F + (RT > 0 ? 1 : 0);
     constexpr int KP = CT * 16;
-    // packed bytes per chunk: 128 where two LDS buffers of its Q8 words
-    // (CB * VT * 64 bytes each) fit the 160 KB of LDS, else 64
-    constexpr int CB = (VT <= 10) ? 128 : 64;
+    // CDMA (8 waves, 6..8 column tiles: one workgroup per CU by registers
+    // anyway): chunks of 64 packed bytes, the codes copied into LDS beside
+    // the Q8 words, three buffers with chunk i+2's copies in flight while
+    // chunk i is multiplied.  Otherwise packed bytes per chunk: 128 where
+    // two LDS buffers of its Q8 words (CB * VT * 64 bytes each) fit the 160
+    // KB of LDS, else 64, and the codes loaded into registers a chunk ahead.
+    constexpr bool CDMA = WAVES == 8 && VT >= 6 && VT <= 8;
+    constexpr int CB = CDMA ? 64 : (VT <= 10) ? 128 : 64;
+    constexpr int NBUF = CDMA ? 3 : 2;
     constexpr int NW = CB / 64;                 // 16-byte words per lane and row
     constexpr int NQT = (CB / 4) * VT * 16;     // Q8 words per chunk
     constexpr int NT = 64 * WAVES;              // threads
     constexpr int NQ = (NQT + NT - 1) / NT;
-    __shared__ u32x4 qs[2][NQT];               // double buffer, one barrier per chunk
+    constexpr int ROWS = WAVES * ST * 16;       // SNP rows per workgroup
+    constexpr int NCT = CDMA ? ROWS * (CB / 16) : 1;  // code words per chunk
+    __shared__ u32x4 qs[NBUF][NQT];            // one barrier per chunk
+    __shared__ u32x4 cs[CDMA ? NBUF : 1][NCT];
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int r = lane & 15, g = lane >> 4;
     int bx, by;  // SNP row block, sample segment
@@ -733,14 +742,34 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
     static_assert(NQT % 64 == 0, "whole waves of Q8 words");
     const uint32_t qs_lds = lds_addr(&qs[0][0]);
     const int wave_u = __builtin_amdgcn_readfirstlane(wave);
+    __builtin_assume(wave_u >= 0 && wave_u < WAVES);
     u32x4 rp[ST][NW];
+    // CDMA: every wave issues NQ copies (one past the end copies word w mod
+    // NQT again, the same bytes to the same place), so that its count of
+    // copies in flight is a constant
 #define DPCA_ZA8_LOADQ(c0, buf)                                                             \
     do {                                                                                    \
         const u32x4 *src_ = Q8 + ((c0) / 4) * VT * 16;                                      \
         _Pragma("unroll") for (int u = 0; u < NQ; u++) {                                   \
-            const int w_ = NT * u + 64 * wave_u;                                            \
+            int w_ = NT * u + 64 * wave_u;                                                  \
+            if (CDMA) w_ %= NQT;                                                            \
             if (w_ < NQT)                                                                   \
                 lds_dma16(qs_lds + 16u * (uint32_t)((buf) * NQT + w_), src_ + w_ + lane);   \
+        }                                                                                   \
+    } while (0)
+    // CDMA: code word w of a chunk is bytes 16 (w % 4) .. +15 of the chunk in
+    // the workgroup's row w / 4, at cs[buf][w]
+    constexpr int NCW = CDMA ? NCT / NT : 0;    // code copies per wave
+    static_assert(!CDMA || NCT % NT == 0, "uniform code copies");
+    const uint32_t cs_lds = lds_addr(&cs[0][0]);
+    const uint8_t *cblk = packed + (int64_t)bx * ROWS * ldp;
+#define DPCA_ZA8_LOADC(c0, buf)                                                             \
+    do {                                                                                    \
+        _Pragma("unroll") for (int u = 0; u < NCW; u++) {                                  \
+            const int w_ = NT * u + 64 * wave_u;                                            \
+            const int wl_ = w_ + lane;                                                      \
+            lds_dma16(cs_lds + 16u * (uint32_t)((buf) * NCT + w_),                          \
+                      cblk + (int64_t)(wl_ / 4) * ldp + (c0) + 16 * (wl_ % 4));             \
         }                                                                                   \
     } while (0)
 #define DPCA_ZA8_LOADP(c0)                                                                  \
@@ -751,10 +780,18 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
                                                  (CB / 4) * g + 16 * n);                    \
     } while (0)
     DPCA_ZA8_LOADQ(b0 < b1 ? b0 : 0, 0);
-    DPCA_ZA8_LOADP(b0 < b1 ? b0 : 0);
+    if constexpr (CDMA) {
+        const int64_t c0 = b0 < b1 ? b0 : 0;
+        const int64_t c1 = (c0 + CB < b1) ? c0 + CB : c0;
+        DPCA_ZA8_LOADC(c0, 0);
+        DPCA_ZA8_LOADQ(c1, 1);
+        DPCA_ZA8_LOADC(c1, 1);
+    } else {
+        DPCA_ZA8_LOADP(b0 < b1 ? b0 : 0);
+    }
     // where the registers allow, the packed codes (HBM) run two chunks
     // ahead and the Q8 words (L2) one
-    constexpr bool PF2 = ST * VT < 16;
+    constexpr bool PF2 = !CDMA && ST * VT < 16;
     u32x4 rp2[ST][NW];
     if constexpr (PF2) {
         const int64_t c2 = (b0 + CB < b1) ? b0 + CB : b0;
@@ -765,17 +802,30 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
                 rp2[s][n] = *reinterpret_cast<const u32x4 *>(packed + (base + s * 16 + r) * ldp + c2 +
                                                              (CB / 4) * g + 16 * n);
     }
-    lds_dma_wait();
+    if constexpr (CDMA) lds_dma_wait_all_but<NQ + NCW>();  // chunk 1 may be in flight
+    else lds_dma_wait();
     __syncthreads();
 
     int cur = 0;
-    for (int64_t cb = b0; cb < b1; cb += CB, cur ^= 1) {
+    for (int64_t cb = b0; cb < b1; cb += CB, cur = (cur + 1 == NBUF) ? 0 : cur + 1) {
         u32x4 pk[ST][NW];
+        if constexpr (CDMA) {
+            // chunk i+2 (the last chunk again near the end) into the buffer
+            // chunk i-1 was read from before the previous barrier
+            const int64_t n2 = (cb + 2 * CB < b1) ? cb + 2 * CB : b1 - CB;
+            const int nb = (cur == 0) ? 2 : cur - 1;
+            DPCA_ZA8_LOADQ(n2, nb);
+            DPCA_ZA8_LOADC(n2, nb);
 #pragma unroll
-        for (int s = 0; s < ST; s++)
+            for (int s = 0; s < ST; s++)
+                pk[s][0] = cs[cur][(wave * ST * 16 + s * 16 + r) * (CB / 16) + g];
+        } else {
 #pragma unroll
-            for (int n = 0; n < NW; n++) pk[s][n] = rp[s][n];
-        {
+            for (int s = 0; s < ST; s++)
+#pragma unroll
+                for (int n = 0; n < NW; n++) pk[s][n] = rp[s][n];
+        }
+        if constexpr (!CDMA) {
             // unconditional (re-reads the last chunk at the end)
             const int64_t nx = (cb + CB < b1) ? cb + CB : cb;
             DPCA_ZA8_LOADQ(nx, cur ^ 1);  // last read before the previous barrier
@@ -839,9 +889,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
             // bunch the VALU work); only SALU moves across
             __builtin_amdgcn_sched_barrier(0x4);
         }
-        lds_dma_wait();
+        if constexpr (CDMA) lds_dma_wait_all_but<NQ + NCW>();  // chunk i+1 landed
+        else lds_dma_wait();
         __syncthreads();
     }
+#undef DPCA_ZA8_LOADC
 #undef DPCA_ZA8_LOADP
 #undef DPCA_ZA8_LOADQ
     float *out = Tpart + (int64_t)by * Npad * KP;
