@@ -529,7 +529,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         // latency hides behind the matrix cores instead of stalling them.
         constexpr int NKS = SC / 32;            // K steps per chunk
         constexpr int NB = NKS * VT;            // B tiles per chunk
-        constexpr int ZB = 4;                   // B tiles in flight
+        // B tiles in flight: 12 with 8-wave workgroups (measured at 100k x
+        // 1M, nrhs 40: 4 -> 8 -> 12 -> 16 tiles, 16.0 -> 15.6 -> 15.1 ->
+        // 15.2 ms; no spills), 4 in the 4-wave tuning forms
+        constexpr int ZB = WAVES == 8 ? (NB < 12 ? NB : 12) : 4;
         const u32x4 *ub = &uw[cur][g * VT * 16 + r];
         auto boff = [](int i) constexpr { return ((i / VT) * 4 * VT + i % VT) * 16; };
         u32x4 bq[ZB];
@@ -847,7 +850,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
         // read from LDS ZB tiles ahead of their MFMAs (the LDS latency then
         // hides behind the matrix cores instead of stalling them).
         constexpr int NB = 4 * NW * VT;         // B tiles per chunk
-        constexpr int ZB = 4;                   // B tiles in flight
+        // B tiles in flight: 12 with 8-wave workgroups (14.5 -> 14.0-14.1 ms
+        // at 100k x 1M, nrhs 40; no spills), 4 in the 4-wave tuning forms
+        constexpr int ZB = WAVES == 8 ? (NB < 12 ? NB : 12) : 4;
         const u32x4 *qb = &qs[cur][(CB / 16) * g * VT * 16 + r];
         u32x4 bq[ZB];
 #pragma unroll
