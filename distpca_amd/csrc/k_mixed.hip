@@ -850,9 +850,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
         // read from LDS ZB tiles ahead of their MFMAs (the LDS latency then
         // hides behind the matrix cores instead of stalling them).
         constexpr int NB = 4 * NW * VT;         // B tiles per chunk
-        // B tiles in flight: 12 with 8-wave workgroups (14.5 -> 14.0-14.1 ms
-        // at 100k x 1M, nrhs 40; no spills), 4 in the 4-wave tuning forms
-        constexpr int ZB = WAVES == 8 ? (NB < 12 ? NB : 12) : 4;
+        // B tiles in flight: 12 with 8-wave workgroups of 6 or more column
+        // tiles (14.5 -> 14.0-14.1 ms at 100k x 1M, nrhs 40; no spills), 4
+        // with fewer (20k x 500k, nrhs 20, 4 tiles: 0.97 ms against 1.01 ms
+        // with 12) and in the 4-wave tuning forms
+        constexpr int ZB = (WAVES == 8 && VT >= 6) ? (NB < 12 ? NB : 12) : 4;
         const u32x4 *qb = &qs[cur][(CB / 16) * g * VT * 16 + r];
         u32x4 bq[ZB];
 #pragma unroll
