@@ -117,8 +117,9 @@ __device__ __forceinline__ i32x4 mfma_i8(const u32x4 &a, const u32x4 &b, const i
 }
 
 // Slice operands reach LDS without passing through registers: one
-// global_load_lds_dwordx4 copies 16 bytes per lane from src into LDS at
-// lds + 16 lane, lds being the wave's (uniform) LDS byte address, set in M0.
+// global_load_lds_dwordx4 copies 16 bytes per lane from base + off into LDS
+// at lds + 16 lane, lds being the wave's (uniform) LDS byte address, set in
+// M0.
 // The compiler does not see this write, so (a) nothing in the kernel stores
 // to the destination buffer through ordinary code, (b) the buffer is read
 // only after the issuing waves have waited for their copies (lds_dma_wait)
@@ -128,9 +129,11 @@ __device__ __forceinline__ i32x4 mfma_i8(const u32x4 &a, const u32x4 &b, const i
 // the compiler and not otherwise used by the kernels that call this (no
 // message, GWS, indexed-move or compiler-issued LDS-DMA instructions), so it
 // is written here without being declared.
-__device__ __forceinline__ void lds_dma16(uint32_t lds, const void *src)
+// The source is a wave-uniform base (an SGPR pair) plus a 32-bit per-lane
+// byte offset, so one offset register serves every copy of a kernel.
+__device__ __forceinline__ void lds_dma16(uint32_t lds, const void *base, uint32_t off)
 {
-    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, off" : : "s"(lds), "v"(src) : "memory");
+    asm volatile("s_mov_b32 m0, %0\n\tglobal_load_lds_dwordx4 %1, %2" : : "s"(lds), "v"(off), "s"(base) : "memory");
 }
 // all of this wave's global loads, the LDS copies among them, have landed
 __device__ __forceinline__ void lds_dma_wait() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
@@ -440,6 +443,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     const uint32_t uw_lds = lds_addr(&uw[0][0]);
     const int wave_u = __builtin_amdgcn_readfirstlane(wave);
     __builtin_assume(wave_u >= 0 && wave_u < WAVES);
+    const uint32_t loff = 16u * (uint32_t)lane;  // per-lane source offset of a copy
 #define DPCA_ZT8_LOADU(c0, buf)                                                          \
     do {                                                                                    \
         const u32x4 *q_ = UW8 + ((c0) / 8) * VT * 16;                                       \
@@ -447,7 +451,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
             int w_ = NT * u + 64 * wave_u;                                                  \
             if (NBUF == 3) w_ %= NUWT;                                                      \
             if (w_ < NUWT)                                                                  \
-                lds_dma16(uw_lds + 16u * (uint32_t)((buf) * NUWT + w_), q_ + w_ + lane);    \
+                lds_dma16(uw_lds + 16u * (uint32_t)((buf) * NUWT + w_), q_ + w_, loff);     \
         }                                                                                   \
     } while (0)
     // 4x4 byte transpose of each block into [byte column][SNP group]
@@ -479,7 +483,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         _Pragma("unroll") for (int u = 0; u < (NIW + NT - 1) / NT; u++) {                  \
             const int w_ = NT * u + 64 * wave_u;                                            \
             if (w_ < NIW)                                                                   \
-                lds_dma16(pk_lds + 16u * (uint32_t)((buf) * NIW + w_), b_ + 16 * (w_ + lane)); \
+                lds_dma16(pk_lds + 16u * (uint32_t)((buf) * NIW + w_), b_ + 16 * w_, loff);    \
         }                                                                                   \
     } while (0)
     // copies per wave and chunk (three buffers: both counts uniform)
@@ -746,6 +750,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
     const uint32_t qs_lds = lds_addr(&qs[0][0]);
     const int wave_u = __builtin_amdgcn_readfirstlane(wave);
     __builtin_assume(wave_u >= 0 && wave_u < WAVES);
+    const uint32_t loff = 16u * (uint32_t)lane;  // per-lane source offset of a Q8 copy
     u32x4 rp[ST][NW];
     // CDMA: every wave issues NQ copies (one past the end copies word w mod
     // NQT again, the same bytes to the same place), so that its count of
@@ -757,7 +762,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
             int w_ = NT * u + 64 * wave_u;                                                  \
             if (CDMA) w_ %= NQT;                                                            \
             if (w_ < NQT)                                                                   \
-                lds_dma16(qs_lds + 16u * (uint32_t)((buf) * NQT + w_), src_ + w_ + lane);   \
+                lds_dma16(qs_lds + 16u * (uint32_t)((buf) * NQT + w_), src_ + w_, loff);    \
         }                                                                                   \
     } while (0)
     // CDMA: code word w of a chunk is bytes 16 (w % 4) .. +15 of the chunk in
@@ -766,13 +771,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
     static_assert(!CDMA || NCT % NT == 0, "uniform code copies");
     const uint32_t cs_lds = lds_addr(&cs[0][0]);
     const uint8_t *cblk = packed + (int64_t)bx * ROWS * ldp;
+    // per-lane offset of a code copy (64 | w, so row w / 4 + lane / 4, bytes
+    // 16 (lane % 4)); < 2^32: the workgroup's rows span ROWS x ldp bytes
+    const uint32_t coff = (uint32_t)((lane / 4) * ldp + 16 * (lane % 4));
 #define DPCA_ZA8_LOADC(c0, buf)                                                             \
     do {                                                                                    \
         _Pragma("unroll") for (int u = 0; u < NCW; u++) {                                  \
             const int w_ = NT * u + 64 * wave_u;                                            \
-            const int wl_ = w_ + lane;                                                      \
             lds_dma16(cs_lds + 16u * (uint32_t)((buf) * NCT + w_),                          \
-                      cblk + (int64_t)(wl_ / 4) * ldp + (c0) + 16 * (wl_ % 4));             \
+                      cblk + (int64_t)(w_ / 4) * ldp + (c0), coff);                         \
         }                                                                                   \
     } while (0)
 #define DPCA_ZA8_LOADP(c0)                                                                  \
