@@ -377,14 +377,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     constexpr int TB = SA * 16 * WAVES / 4;  // packed bytes per row in this tile
     constexpr int SC = 128;                // SNPs per chunk
     constexpr int NUWT = (SC / 8) * VT * 16;  // UW8 words per chunk
-    // two buffers: chunk i is multiplied out of one while chunk i+1 lands in
-    // the other (one barrier per chunk).  pk rows are byte columns, their
-    // dwords 4-SNP groups, stored at column sg ^ (row / 4) in rows of 36
-    // dwords: both the transposing stores (one group, many rows per
-    // half-wave) and the paired reads (4 rows x 2 groups) then touch
-    // distinct banks.  Groups 2x, 2x+1 stay one aligned 8-byte pair, in
-    // swapped order when row / 4 is odd.  IMG: rows of 32 dwords in the
-    // image's order (pairs never swapped).
+    // two buffers (three in most IMG forms, below): chunk i is multiplied
+    // out of one while chunk i+1 lands in the other (one barrier per chunk).
+    // pk rows are byte columns, their dwords 4-SNP groups, stored at column
+    // sg ^ (row / 4) in rows of 36 dwords: both the transposing stores (one
+    // group, many rows per half-wave) and the paired reads (4 rows x 2
+    // groups) then touch distinct banks.  Groups 2x, 2x+1 stay one aligned
+    // 8-byte pair, in swapped order when row / 4 is odd.  IMG: rows of 32
+    // dwords in the image's order (pairs never swapped).
     constexpr int PKW = IMG ? SC / 4 : SC / 4 + 4;
     // IMG, where the registers hold one workgroup per CU anyway and three
     // buffers fit the LDS: three buffers, chunk i+2's copies in flight while
@@ -678,7 +678,8 @@ hipError_t launch_q8(hipStream_t st, const double *Q, int64_t Mpad, int k, int n
 // Workgroup: WAVES waves x ST tiles x 16 SNPs, one sample segment (blockIdx.y).
 // Per chunk of CB packed bytes per row (4 CB samples), lane group g owns the
 // CB samples of its CB / 4 packed bytes; at K step t it supplies samples
-// CB g + 16t .. +15.  The chunk's Q8 words are staged in LDS.
+// CB g + 16t .. +15.  The chunk's Q8 words (CDMA, below: and its codes) are
+// copied into LDS.
 template <int NS, int ST, int F, int RT, bool SKR, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict__ packed, int64_t ldp,
                                                const u32x4 *__restrict__ Q8,
