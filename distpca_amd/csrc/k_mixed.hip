@@ -444,7 +444,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     const int wave_u = __builtin_amdgcn_readfirstlane(wave);
     __builtin_assume(wave_u >= 0 && wave_u < WAVES);
     const uint32_t loff = 16u * (uint32_t)lane;  // per-lane source offset of a copy
-#define DPCA_ZT8_LOADU(c0, buf)                                                          \
+#define DPCA_ZT8_LOADU(c0, buf)                                                             \
     do {                                                                                    \
         const u32x4 *q_ = UW8 + ((c0) / 8) * VT * 16;                                       \
         _Pragma("unroll") for (int u = 0; u < NUW; u++) {                                  \
@@ -597,6 +597,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
         else lds_dma_wait();
         __syncthreads();
     }
+    lds_dma_wait();  // no copy into this workgroup's LDS outlives it
 #undef DPCA_ZT8_STORE
 #undef DPCA_ZT8_LOADI
 #undef DPCA_ZT8_LOADU
@@ -908,6 +909,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
         else lds_dma_wait();
         __syncthreads();
     }
+    lds_dma_wait();  // no copy into this workgroup's LDS outlives it
 #undef DPCA_ZA8_LOADC
 #undef DPCA_ZA8_LOADP
 #undef DPCA_ZA8_LOADQ
