@@ -135,6 +135,35 @@ def test_zt_image_not_used(monkeypatch):
         assert rep["zt_image"] == used, (kw, waves, rep["zt_image"])
 
 
+@pytest.mark.parametrize("N", [1536, 2560, 3584])
+def test_three_buffer_short_ranges(N, capfd, monkeypatch):
+    # the three-buffer forms of both int8 products (chunk i+2's copies in
+    # flight, the last chunk copied again near its range's end) where SNP
+    # splits and sample segments hold only a few chunks: against the oracle,
+    # and Z^T T from the tile-order copy bitwise against the staging kernel.
+    # On an MI355X the plan takes 6 splits of 384 / 512 / 768 SNPs (3 / 4 / 6
+    # chunks of 128; last splits of 1 and 2 chunks) and sample segments of
+    # 256 / 384 / 512 packed bytes (4 / 6 / 8 chunks of 64; a last one of 2)
+    M, k = 100000, 40
+    print(f"[gpu] three-buffer forms, short ranges, M={M} N={N}", flush=True)
+    packed = _packed_random(M, N, seed=N)
+    Q = np.linalg.qr(np.random.default_rng(N).standard_normal((M, k)))[0]
+    ref = O.OOCPass(packed, M, N, 1024)(Q)
+    monkeypatch.setenv("DPCA_SHOW_PLAN", "1")
+    capfd.readouterr()
+    out = {}
+    for img in (0, 1):
+        p = D.Params(nsv=4, nrhs=k, bsize=1024, precision=D.MIXED, zt_image=img)
+        with D.PCA.from_packed(packed, M, N, p) as pca:
+            out[img] = pca.pass_(Q)
+    plans = [l for l in capfd.readouterr().err.splitlines() if l.startswith("dpca plan:")]
+    print(plans[-1])
+    err = np.abs(out[1] - ref).max() / np.abs(ref).max()
+    print("pass rel err", err)
+    assert err < 2e-5, err
+    assert np.array_equal(out[0].view(np.int64), out[1].view(np.int64))
+
+
 def test_cli_ztimage_flag(tmp_path):
     # DistPCA.exe -precision mixed with and without the tile-order copy
     # (-ztimage 0) writes the same files, byte for byte
