@@ -136,16 +136,20 @@ def cpu_bsize(M, N, nsv, nrhs, bsize):
     return O.resolve(O.Params(nsv=nsv, nrhs=nrhs, bsize=bsize, memory=16.0), M, N).bsize
 
 
-def cpu_baseline(pca, M, N, k, bsize, gpu_passes=None, target_s=15.0):
+def cpu_baseline(pca, M, N, k, bsize, gpu_passes=None, target_s=15.0, host_bytes=0):
     """The oracle's pass (the reference's serial cached decode,
     src/methods.cpp:838-843, plus threaded host dgemm) timed on a bounded SNP
     sample of the same matrix; seconds per full pass and per full run are
-    extrapolated from it and labelled so."""
+    extrapolated from it and labelled so.  The sample's decoded FP64 cache
+    (rows x M x 8 bytes) stays within 64 GB less what the workload already
+    holds in host memory (host_bytes: a matrix streamed from the host), and
+    at least 3.2 GB (1,000 SNPs at 400,000 samples)."""
     from oracle import oracle as O
     info = host_cpu_info()
     np_ = (M + 3) // 4
     Q = np.linalg.qr(np.random.default_rng(0).standard_normal((M, k)))[0]
-    rows = min(N, 1024)
+    max_rows = max(256, int(max(3.2e9, 64e9 - host_bytes) / (8.0 * M)))
+    rows = min(N, 1024, max_rows)
     while True:
         packed = pca.packed(0, rows)
         op = O.OOCPass(packed.reshape(-1), M, rows, min(rows, bsize))
@@ -154,9 +158,9 @@ def cpu_baseline(pca, M, N, k, bsize, gpu_passes=None, target_s=15.0):
         t0 = time.perf_counter()
         op(Q)                                    # cached pass, as later passes
         dt = time.perf_counter() - t0
-        if dt * 2 > target_s or rows >= N or rows >= 65536:
+        if dt * 2 > target_s or rows >= N or rows >= min(65536, max_rows):
             break
-        rows = min(N, 65536, rows * max(2, int(target_s / max(dt, 1e-3) / 2)))
+        rows = min(N, 65536, max_rows, rows * max(2, int(target_s / max(dt, 1e-3) / 2)))
     gbps = rows * np_ / dt / 1e9
     pass_s = dt * N / rows
     out = {"value": round(gbps, 4), "unit": "GB/s", "cores": info["blas_threads"], "kind": "port",
@@ -514,8 +518,10 @@ def main():
     cpu = None
     if not a.no_cpu:
         log("cpu baseline (oracle pass on a bounded sample)")
+        # a matrix streamed from host memory (not from a file) holds its bytes there
+        host_bytes = packed_bytes if (cap and not bed) else 0
         cpu = cpu_baseline(mixed, M, N, nrhs, cpu_bsize(M, N, nsv, nrhs, bsize),
-                           extra.get("mixed_pca_passes"), a.cpu_seconds)
+                           extra.get("mixed_pca_passes"), a.cpu_seconds, host_bytes)
     mixed.close()
 
     if not a.no_fp64:
