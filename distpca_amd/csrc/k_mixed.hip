@@ -692,13 +692,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_za_i8(const uint8_t *__restrict_
     constexpr int VT = NS * F + RT;
     constexpr int CT = F + (RT > 0 ? 1 : 0);
     constexpr int KP = CT * 16;
-    // CDMA (8 waves, 6..8 column tiles: one workgroup per CU by registers
-    // anyway): chunks of 64 packed bytes, the codes copied into LDS beside
-    // the Q8 words, three buffers with chunk i+2's copies in flight while
-    // chunk i is multiplied.  Otherwise packed bytes per chunk: 128 where
-    // two LDS buffers of its Q8 words (CB * VT * 64 bytes each) fit the 160
-    // KB of LDS, else 64, and the codes loaded into registers a chunk ahead.
-    constexpr bool CDMA = WAVES == 8 && VT >= 6 && VT <= 8;
+    // CDMA (8 waves, 5..8 column tiles: the resident workgroups per CU are
+    // set by the registers anyway): chunks of 64 packed bytes, the codes
+    // copied into LDS beside the Q8 words, three buffers with chunk i+2's
+    // copies in flight while chunk i is multiplied (100k x 1M: 3 slices, 8
+    // tiles, 14.9 -> 14.4 ms; 2 slices, 5 tiles, 10.0 -> 9.9 ms; at 4 tiles
+    // no change).  Otherwise packed bytes per chunk: 128 where two LDS
+    // buffers of its Q8 words (CB * VT * 64 bytes each) fit the 160 KB of
+    // LDS, else 64, and the codes loaded into registers a chunk ahead.
+    constexpr bool CDMA = WAVES == 8 && VT >= 5 && VT <= 8;
     constexpr int CB = CDMA ? 64 : (VT <= 10) ? 128 : 64;
     constexpr int NBUF = CDMA ? 3 : 2;
     constexpr int NW = CB / 64;                 // 16-byte words per lane and row
