@@ -386,11 +386,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_zt_i8(const uint8_t *__restrict_
     // 8-byte pair, in swapped order when row / 4 is odd.  IMG: rows of 32
     // dwords in the image's order (pairs never swapped).
     constexpr int PKW = IMG ? SC / 4 : SC / 4 + 4;
-    // IMG, where the registers hold one workgroup per CU anyway and three
-    // buffers fit the LDS: three buffers, chunk i+2's copies in flight while
-    // chunk i is multiplied (twice the time to cover HBM latency)
+    // IMG with 4 sample tiles per wave and 4 or more column tiles (the
+    // registers hold one workgroup per CU anyway), where three buffers fit
+    // the LDS: three buffers, chunk i+2's copies in flight while chunk i is
+    // multiplied (twice the time to cover HBM latency; 0.5-1.2% faster)
     constexpr int BUFB = TB * PKW * 4 + NUWT * 16;  // LDS bytes per buffer
-    constexpr int NBUF = (IMG && SA * VT >= 24 && 3 * BUFB <= 160 * 1024) ? 3 : 2;
+    constexpr int NBUF = (IMG && SA >= 4 && VT >= 4 && 3 * BUFB <= 160 * 1024) ? 3 : 2;
     static_assert(SA % 2 == 0, "row / 4 parity must follow the tile index");
     __shared__ uint32_t pk[NBUF][TB][PKW];
     __shared__ u32x4 uw[NBUF][NUWT];
