@@ -263,33 +263,61 @@ __global__ void k_split_max(const double *__restrict__ T, int64_t rows, int kp,
 // word {U_j(8G..+3), U_j(8G+4..+7), W_j(8G..+3), W_j(8G+4..+7)} of the slice
 // and column of (v, r), at UW8[(G * VT + v) * 16 + r].
 
+// One thread per (8-SNP group, column of the virtual tiles): the column's
+// NS slices of U and W for the 8 SNPs go to the NS words of their virtual
+// tiles; the padding slots of the shared remainder tiles get zero words.
+// Columns c < 16 F sit in whole tiles (slice j at tile NS (c / 16) + j, lane
+// c % 16); the R columns past them share the remainder tiles (slice j of
+// column 16 F + x at virtual column 16 NS F + j R + x).
 template <int NS>
 __global__ void k_uw8(const double *__restrict__ T, int64_t rows, int kp, int F, int R, int VT,
                       const double *__restrict__ mu, const double *__restrict__ dsc, int64_t rps,
                       const unsigned *__restrict__ smax, u32x4 *__restrict__ UW8)
 {
-    DPCA_EW_LOOP(idx, rows / 8 * VT * 16) {
-        const int64_t G = idx / (VT * 16);
-        const int v = (int)(idx % (VT * 16)) / 16, r = (int)(idx % 16);
-        int j, c;
-        vcol<NS>(v, r, F, R, j, c);
-        uint32_t w[4] = {0, 0, 0, 0};
-        if (c >= 0) {
-            const int64_t split = (G * 8) / rps;
-            const double inv = 1.0 / slice_scale(__uint_as_float(smax[split * kp + c]));
+    const int CW = 16 * F + R;                 // columns with slices
+    const int PAD = (VT - NS * F) * 16 - NS * R;  // empty slots of the remainder tiles
+    const int per = CW + PAD;
+    DPCA_EW_LOOP(idx, rows / 8 * per) {
+        const int64_t G = idx / per;
+        const int c = (int)(idx % per);
+        u32x4 *out = UW8 + G * VT * 16;
+        if (c >= CW) {  // a padding slot
+            const int w = NS * R + (c - CW);
+            out[(NS * F + w / 16) * 16 + w % 16] = u32x4{0, 0, 0, 0};
+            continue;
+        }
+        const int64_t split = (G * 8) / rps;
+        const double inv = 1.0 / slice_scale(__uint_as_float(smax[split * kp + c]));
+        uint32_t wu[NS][2], ww[NS][2];
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int64_t row = G * 8 + e;
-                const double t = T[row * kp + c];
-                int qu[NS], qw[NS];
-                slices<NS>(dsc[row] * t, inv, qu);
-                slices<NS>(-(dsc[row] * mu[row]) * t, inv, qw);
-                const int sh = 8 * (e & 3);
-                w[e >> 2] |= (uint32_t)(qu[j] & 0xFF) << sh;
-                w[2 + (e >> 2)] |= (uint32_t)(qw[j] & 0xFF) << sh;
+        for (int j = 0; j < NS; j++) wu[j][0] = wu[j][1] = ww[j][0] = ww[j][1] = 0;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int64_t row = G * 8 + e;
+            const double t = T[row * kp + c];
+            int qu[NS], qw[NS];
+            slices<NS>(dsc[row] * t, inv, qu);
+            slices<NS>(-(dsc[row] * mu[row]) * t, inv, qw);
+            const int sh = 8 * (e & 3);
+#pragma unroll
+            for (int j = 0; j < NS; j++) {
+                wu[j][e >> 2] |= (uint32_t)(qu[j] & 0xFF) << sh;
+                ww[j][e >> 2] |= (uint32_t)(qw[j] & 0xFF) << sh;
             }
         }
-        UW8[idx] = u32x4{w[0], w[1], w[2], w[3]};
+#pragma unroll
+        for (int j = 0; j < NS; j++) {
+            int v, r;
+            if (c < 16 * F) {
+                v = (c / 16) * NS + j;
+                r = c % 16;
+            } else {
+                const int w = j * R + (c - 16 * F);
+                v = NS * F + w / 16;
+                r = w % 16;
+            }
+            out[v * 16 + r] = u32x4{wu[j][0], wu[j][1], ww[j][0], ww[j][1]};
+        }
     }
 }
 
@@ -309,7 +337,7 @@ hipError_t launch_uw8(hipStream_t st, const double *T, int64_t rows, int k, int 
         dim3 g1((unsigned)((rps + 127) / 128), (unsigned)nsplit);
         k_split_max<<<g1, 4 * kp, 0, st>>>(T, rows, kp, mu, dsc, rps, smax);
     }
-    const int64_t n = rows / 8 * ct.vt() * 16;
+    const int64_t n = rows / 8 * ((ct.vt() - ns * ct.F) * 16 + 16 * ct.F + ct.R - ns * ct.R);
     if (ns == 2)
         k_uw8<2><<<ew_blocks(n), 256, 0, st>>>(T, rows, kp, ct.F, ct.R, ct.vt(), mu, dsc, rps, smax,
                                                 reinterpret_cast<u32x4 *>(UW8));
