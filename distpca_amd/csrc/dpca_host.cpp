@@ -82,7 +82,11 @@ static inline int32_t seed_of(int64_t e, int M, int k)
 
 static void gen_range(int64_t e0, int64_t e1, int M, int k, long state, double *out)
 {
+    // no fused multiply-adds (the reference's IEEE operation order): g++ gets
+    // -ffp-contract=off from the build, clang this pragma
+#ifdef __clang__
 #pragma clang fp contract(off)
+#endif
     Lcg g;
     g.x = state;
     const double pi = 3.14159265;
